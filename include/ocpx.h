@@ -1,0 +1,267 @@
+/*
+ * ocpx.h — C ABI of the MI355X flash + FIM-assembly hot path.
+ *
+ * One context = one reservoir region on one GPU, bound to one HIP stream.
+ * The context owns a device-resident structure-of-arrays mirror of the
+ * reference's BulkVarSet / BulkConnVarSet (include/BulkVarSet.hpp:34-294,
+ * include/BulkConnVarSet.hpp:27-172) and a device block-CSR matrix that
+ * replaces OCPMatrix (include/OCPMatrix.hpp:26-101).
+ *
+ * Units follow the reference's METRIC system (include/OCPUnits.hpp:78-100):
+ * pressure in bar, temperature in degC in the state arrays (the flash adds
+ * 273.15, src/OCPMixtureMethodComp.cpp:674), length in m, permeability in
+ * mD, time in days.  Arithmetic is FP64 throughout.
+ *
+ * Naming: Nc = number of hydrocarbon (Peng-Robinson) components, nc = Nc+1
+ * bulk components (water is always the last component), np = 3 phases in
+ * the order oil=0, gas=1, water=2, block size b = nc+1.
+ *
+ * Every entry point returns 0 on success and a negative OCPX_E_* code on
+ * failure; ocpx_last_error() then describes it.  Nothing aborts.
+ * Calls are asynchronous on the context's stream unless stated otherwise;
+ * ocpx_sync() waits.  A context is thread-compatible, not thread-safe.
+ */
+#ifndef OCPX_H
+#define OCPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OCPX_MAX_NC      16   /* max hydrocarbon components per fluid   */
+#define OCPX_MAX_TABROWS 64   /* max rows of a SWOF / SGOF table        */
+
+/* status codes */
+#define OCPX_OK            0
+#define OCPX_E_ARG        -1   /* bad argument / size                    */
+#define OCPX_E_HIP        -2   /* HIP runtime error                      */
+#define OCPX_E_STATE      -3   /* call order (e.g. assemble before flash) */
+#define OCPX_E_NOMEM      -4
+#define OCPX_E_UNSUPPORTED -5
+
+/*
+ * Fluid description of one PVT region.  Replaces the ComponentParam +
+ * PVTW + flash-control keywords read by ParamReservoir
+ * (src/ParamReservoir.cpp; consumed by EoS_PR::EoS_PR src/OCPEoS.cpp:19-65,
+ * OCPPhaseEquilibrium::Setup src/OCPPhaseEquilibrium.cpp:15-68,
+ * OCPMixtureMethodComp::Setup src/OCPMixtureMethodComp.cpp:25-74,
+ * ViscosityMethod03 src/OCPFuncPVT.cpp:415-459,
+ * OCPMixtureMethodComp01 ctor src/OCPMixtureMethodComp.cpp:650-668).
+ * Arrays are indexed by hydrocarbon component; BIC is full, symmetric,
+ * row-major with a fixed row stride of OCPX_MAX_NC.
+ */
+typedef struct ocpx_fluid {
+    int32_t nc;                      /* Nc, 1..OCPX_MAX_NC               */
+    int32_t lbc_exact;               /* 0: reference LBC d(mu)/dx, whose
+                                        d(sqrt MW) term is 0 through integer
+                                        division (src/OCPFuncPVT.cpp:537);
+                                        1: include that term (exact tangent) */
+    double Tc[OCPX_MAX_NC];          /* K                                */
+    double Pc[OCPX_MAX_NC];          /* bar                              */
+    double Vc[OCPX_MAX_NC];          /* m3/kmol (phase identification)   */
+    double Vcvis[OCPX_MAX_NC];       /* m3/kmol (LBC viscosity)          */
+    double MW[OCPX_MAX_NC];
+    double acf[OCPX_MAX_NC];         /* acentric factor                  */
+    double OmegaA[OCPX_MAX_NC];      /* default 0.457235529              */
+    double OmegaB[OCPX_MAX_NC];      /* default 0.077796074              */
+    double Vshift[OCPX_MAX_NC];      /* dimensionless deck value; scaled
+                                        by R*OmegaB*Tc/Pc internally     */
+    double BIC[OCPX_MAX_NC * OCPX_MAX_NC];
+    double LBCcoef[5];               /* deck values; x10 internally      */
+    double PVTW[5];                  /* Pref, Bw(Pref), Cw, muw(Pref), Cvw */
+    double rhoW_std;                 /* water density at std conditions  */
+    /* flash control (deck keywords RR, SSMSTA, NRSTA, SSMSP, NRSP) */
+    int32_t RR_maxIt, SSMsta_maxIt, NRsta_maxIt, SSMsp_maxIt, NRsp_maxIt;
+    int32_t skipPSA;                 /* 1 = SkipPSA on (reference default) */
+    double RR_tol, SSMsta_tol, SSMsta_eYt, NRsta_tol, SSMsp_tol, NRsp_tol;
+} ocpx_fluid;
+
+/*
+ * Saturation tables for OCPFlowMethod_OGW01 (src/OCPFlowMethod.cpp:19-73):
+ * SWOF rows (Sw, krw, krow, Pcow) and SGOF rows (Sg, krg, krog, Pcgo),
+ * row-major, 4 doubles per row.  Oil kr by Stone II
+ * (OCP3POilPerMethod01, src/OCPFuncSAT.cpp:149-186).
+ */
+typedef struct ocpx_sat {
+    int32_t nswof, nsgof;
+    double swof[4 * OCPX_MAX_TABROWS];
+    double sgof[4 * OCPX_MAX_TABROWS];
+} ocpx_sat;
+
+/* OCPRockIsoT_Linear (src/OCPRock.cpp:19-24) */
+typedef struct ocpx_rock {
+    double Pref, cp1, cp2;
+} ocpx_rock;
+
+/*
+ * Cells and connections.  Connections follow BulkConn::iteratorConn
+ * (src/Reservoir.cpp:456-531): bId < eId, sorted by bId, and per cell in
+ * x+, y+, z+ order, so that each block row's columns come out ascending
+ * with the diagonal block first (OCPMatrix push order,
+ * src/OCPMatrix.cpp:92-137).  trans already includes CONV_DARCY
+ * (BulkConnTransMethod01::CalTrans, src/BulkConnFunc.cpp:15-67).
+ * The arrays are copied at ocpx_ctx_create.
+ */
+typedef struct ocpx_grid {
+    int64_t nb;                      /* cells                            */
+    int64_t nconn;                   /* connections (may be 0)           */
+    const double*  v;                /* bulk volume [nb]                 */
+    const double*  depth;            /* cell-centre depth [nb]           */
+    const double*  poroInit;         /* [nb]                             */
+    const int32_t* conn_b;           /* [nconn]                          */
+    const int32_t* conn_e;           /* [nconn]                          */
+    const double*  trans;            /* [nconn]                          */
+} ocpx_grid;
+
+/* Residual norms of IsoT_FIM::CalRes (src/IsoThermalMethod.cpp:1145-1168) */
+typedef struct ocpx_resnorms {
+    double maxRelRes_V;   int64_t maxId_V;
+    double maxRelRes_N;   int64_t maxId_N;
+} ocpx_resnorms;
+
+/* Per-cell state fields, exchanged in the reference's BulkVarSet layout
+ * (cell-major: field[n*len + k]).  len is given in the comment. */
+enum ocpx_field {
+    OCPX_F_P = 0,         /* [1]   bvs.P                                 */
+    OCPX_F_T,             /* [1]   bvs.T (degC)                          */
+    OCPX_F_NI,            /* [nc]  bvs.Ni                                */
+    OCPX_F_LNI,           /* [nc]  bvs.lNi (Ni at the last time step)    */
+    OCPX_F_PHASEEXIST,    /* [np]  uint32 (OCP_BOOL)                     */
+    OCPX_F_S,             /* [np]                                        */
+    OCPX_F_XIJ,           /* [np*nc]                                     */
+    OCPX_F_RHO,           /* [np]                                        */
+    OCPX_F_XI,            /* [np]                                        */
+    OCPX_F_MU,            /* [np]                                        */
+    OCPX_F_RHOP,          /* [np]                                        */
+    OCPX_F_XIP,           /* [np]                                        */
+    OCPX_F_MUP,           /* [np]                                        */
+    OCPX_F_RHOX,          /* [np*nc]                                     */
+    OCPX_F_XIX,           /* [np*nc]                                     */
+    OCPX_F_MUX,           /* [np*nc]                                     */
+    OCPX_F_NT,            /* [1]                                         */
+    OCPX_F_VF,            /* [1]                                         */
+    OCPX_F_VFP,           /* [1]                                         */
+    OCPX_F_VFI,           /* [nc]                                        */
+    OCPX_F_DSEC_DPRI,     /* [np*(nc+1)*(nc+1)]  dXsdXp, row-major       */
+    OCPX_F_KR,            /* [np]                                        */
+    OCPX_F_PC,            /* [np]                                        */
+    OCPX_F_PJ,            /* [np]                                        */
+    OCPX_F_DKRDS,         /* [np*np]                                     */
+    OCPX_F_DPCDS,         /* [np*np]                                     */
+    OCPX_F_PORO,          /* [1]                                         */
+    OCPX_F_POROP,         /* [1]                                         */
+    OCPX_F_ROCKVP,        /* [1]                                         */
+    OCPX_F_FTYPE_IN,      /* [1]   int32; -1 = let SkipPSA decide        */
+    OCPX_F_FTYPE_OUT,     /* [1]   int32; PE ftype after the flash       */
+    OCPX_F_SKIP_FLAG,     /* [1]   uint32 SkipPSAVarset::flag            */
+    OCPX_F_SKIP_MINEIG,   /* [1]   SkipPSAVarset::minEigen               */
+    OCPX_F_SKIP_P,        /* [1]                                         */
+    OCPX_F_SKIP_T,        /* [1]                                         */
+    OCPX_F_SKIP_ZI,       /* [Nc]                                        */
+    OCPX_F_ITERS,         /* [8]   int32 per-cell counters, see below    */
+    OCPX_F_NFIELDS
+};
+
+/* OCPX_F_ITERS layout: SSMsta iters, NRsta iters, SSMsp iters, NRsp iters,
+ * RR iters, fugacity evaluations, ln-fugacity-derivative evaluations,
+ * dense solves. */
+
+typedef struct ocpx_ctx ocpx_ctx;
+
+/* Per-cell entry size (doubles or 4-byte ints) of a field for a fluid with
+ * Nc hydrocarbon components. */
+int64_t ocpx_field_len(int field, int32_t Nc);
+
+/* Create a context (≙ IsoT_FIM::Setup/AllocateReservoir,
+ * src/IsoThermalMethod.cpp:711-715, 905-998).  stream may be NULL (default
+ * stream).  Allocates all device memory up front. */
+int  ocpx_ctx_create(const ocpx_fluid* fluid, const ocpx_sat* sat,
+                     const ocpx_rock* rock, const ocpx_grid* grid,
+                     void* hip_stream, ocpx_ctx** out);
+void ocpx_ctx_destroy(ocpx_ctx* ctx);
+const char* ocpx_last_error(const ocpx_ctx* ctx);
+int  ocpx_sync(ocpx_ctx* ctx);
+
+/* Host <-> device state exchange in BulkVarSet layout (synchronous). */
+int  ocpx_set_field(ocpx_ctx* ctx, int field, const void* host);
+int  ocpx_get_field(ocpx_ctx* ctx, int field, void* host);
+/* Device pointer of a field (component-major SoA: f[k*nb + n]). */
+int  ocpx_field_device_ptr(ocpx_ctx* ctx, int field, void** dptr);
+
+/* Flash modes */
+#define OCPX_FLASH_PLAIN    0   /* one cell per lane, one launch          */
+#define OCPX_FLASH_BATCHED  1   /* persistent kernel with work queue      */
+
+/* ≙ IsoT_FIM::CalFlash + PassFlashValue (src/IsoThermalMethod.cpp:1016-1071):
+ * SkipPSA::CalFtype02 (src/AcceleratePEC.cpp:103-124) unless FTYPE_IN >= 0,
+ * OCPMixtureMethodComp01::FlashDer (src/OCPMixtureMethodComp.cpp:746-755),
+ * SkipPSA::CalSkipForNextStep (src/AcceleratePEC.cpp:127-172). */
+int  ocpx_flash_der(ocpx_ctx* ctx, int mode);
+/* Phase equilibrium only (OCPPhaseEquilibrium::PhaseEquilibrium,
+ * src/OCPPhaseEquilibrium.cpp:71-112), for the batched-flash throughput
+ * workload.  Results: OCPX_F_FTYPE_OUT, ITERS and the split below. */
+int  ocpx_phase_equilibrium(ocpx_ctx* ctx, int mode);
+/* NP [nb] int32, nu [nb*2], x [nb*2*Nc] (PE order, before relabelling). */
+int  ocpx_get_pe_result(ocpx_ctx* ctx, int32_t* NP, double* nu, double* x);
+
+/* ≙ IsoT_FIM::CalKrPc (src/IsoThermalMethod.cpp:1073-1088) */
+int  ocpx_krpc_der(ocpx_ctx* ctx);
+/* ≙ IsothermalMethod::CalRock (src/IsoThermalMethod.cpp:19-30) */
+int  ocpx_cal_rock(ocpx_ctx* ctx);
+/* ≙ IsoT_FIM::CalRes (src/IsoThermalMethod.cpp:1090-1185) without wells.
+ * Keeps resAbs (negated, [(nb)*(nc+1)]) on the device; norms and resAbs
+ * are copied to the host when the pointers are non-NULL (synchronous then). */
+int  ocpx_cal_res(ocpx_ctx* ctx, double dt, double* resAbs, ocpx_resnorms* norms);
+/* ≙ IsoT_FIM::AssembleMatBulks + LinearSystem::CopyRhs
+ * (src/IsoThermalMethod.cpp:1187-1269, 742-752) into the device BSR. */
+int  ocpx_assemble(ocpx_ctx* ctx, double dt);
+/* BSR pattern and values (host copies).  rowptr [nb+1], colind [nnzb],
+ * val [nnzb*b*b] (each block row-major), rhs [nb*b].  Any pointer may be
+ * NULL.  nnzb = nb + 2*nconn. */
+int  ocpx_get_bsr(ocpx_ctx* ctx, int64_t* nnzb, int64_t* rowptr,
+                  int32_t* colind, double* val, double* rhs);
+/* Device pointers of the BSR arrays (valid for the context lifetime). */
+int  ocpx_bsr_device_ptrs(ocpx_ctx* ctx, void** rowptr, void** colind,
+                          void** val, void** rhs);
+
+/* ≙ IsoT_FIM::GetSolution (src/IsoThermalMethod.cpp:1282-1418) without
+ * wells or halos: dSmax chop, S/xij/P/Ni update.  u is a host array
+ * [nb*(nc+1)]. */
+int  ocpx_get_solution(ocpx_ctx* ctx, const double* u, double dSmax);
+
+/* One full Newton-step pass: flash_der + krpc + rock + res + assemble.
+ * With use_graph != 0 the pass is captured once into a hipGraph and
+ * replayed afterwards (dt is baked in at capture; re-captured if dt or
+ * mode changes). */
+int  ocpx_newton_pass(ocpx_ctx* ctx, double dt, int mode, int use_graph);
+
+/* Differentiation of the assembled system.
+ * ocpx_jt_lambda: out = J^T lambda, J the BSR assembled last
+ * (lambda, out: host [nb*(nc+1)]).
+ * ocpx_dR_dparams: gradients of lambda . R with respect to the connection
+ * transmissibilities [nconn] and the cell rock pore volumes [nb]
+ * (∂R/∂T from OCPConvection01::CalFlux, src/OCPConvection.cpp:22-79;
+ * ∂R/∂Vp from BulkAccumuTerm01::CalResFIM, src/BulkAccumuModule.cpp:24-44),
+ * sign convention of the un-negated residual R = accumulation + dt*flux. */
+int  ocpx_jt_lambda(ocpx_ctx* ctx, const double* lambda, double* out);
+int  ocpx_dR_dparams(ocpx_ctx* ctx, double dt, const double* lambda,
+                     double* dTrans, double* dVp);
+/* Device-pointer variants of the two adjoint entry points (arrays of the
+ * same sizes in device memory; asynchronous on the context stream). */
+int  ocpx_jt_lambda_dev(ocpx_ctx* ctx, const double* lambda_d, double* out_d);
+int  ocpx_dR_dparams_dev(ocpx_ctx* ctx, double dt, const double* lambda_d,
+                         double* dTrans_d, double* dVp_d);
+
+/* Kernel timing: average duration (ms) of the kernels of one entry point
+ * over the calls since the last reset, from HIP events recorded on the
+ * context stream.  which: 0 PE, 1 flash tail, 2 krpc+rock, 3 residual,
+ * 4 assembly. */
+int  ocpx_timing_enable(ocpx_ctx* ctx, int on);
+int  ocpx_timing_get(ocpx_ctx* ctx, int which, double* avg_ms, int64_t* calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OCPX_H */

@@ -1,0 +1,695 @@
+// ocpx_host.cpp — C ABI (include/ocpx.h) over the gfx950 kernels.
+//
+// The context mirrors what IsoT_FIM owns in the reference
+// (IsoT_FIM::AllocateReservoir, src/IsoThermalMethod.cpp:905-998): the
+// BulkVarSet arrays, the connection list and the linear system, here all
+// device-resident and component-major.  Host arrays cross the boundary in
+// the reference's BulkVarSet layout (cell-major) and are transposed here.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ocpx.h"
+#include "ocpx_dev.hpp"
+#include "ocpx_kernels.hpp"
+
+namespace ocpx {
+int64_t res_asm_blocks(int nc, int64_t nb);
+int pe_batched_blocks_per_cu(int NC);
+}
+
+using namespace ocpx;
+
+namespace {
+
+struct Field {
+    void* d = nullptr;   // device SoA [len][nb]
+    int len = 0;         // entries per cell
+    int esz = 8;         // element size
+};
+
+struct TimingAcc {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0;
+    int64_t calls = 0;
+};
+
+}  // namespace
+
+struct ocpx_ctx {
+    int NC = 0, nc = 0;
+    int64_t nb = 0, nconn = 0, nnzb = 0;
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    std::string err;
+    DevFluid hf{};
+    DevFluid* df = nullptr;
+    DevSat* ds = nullptr;
+    ocpx_rock rock{};
+    Field f[OCPX_F_NFIELDS];
+    bool ftype_override = false;
+    // geometry
+    double *v = nullptr, *depth = nullptr, *poroInit = nullptr, *trans = nullptr;
+    int32_t *cb = nullptr, *ce = nullptr;
+    // PE results
+    int32_t* pe_NP = nullptr;
+    double *pe_nu = nullptr, *pe_x = nullptr;
+    unsigned* queue = nullptr;
+    // linear system
+    int64_t* rowptr = nullptr;
+    int32_t *colind = nullptr, *tslot = nullptr, *nbr_conn = nullptr;
+    double *val = nullptr, *rhs = nullptr, *resAbs = nullptr;
+    double* bmax = nullptr;
+    int64_t* barg = nullptr;
+    double* norms4 = nullptr;
+    double *lam = nullptr, *out = nullptr, *gT = nullptr, *gV = nullptr;
+    bool flashed = false, assembled = false;
+    int grid_waves = 0;
+    // timing
+    bool timing = false;
+    TimingAcc tacc[5];
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double gdt = 0;
+    int gmode = -1;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(ocpx_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                            \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(c, OCPX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+template <class T>
+int dalloc(ocpx_ctx* c, T** p, size_t count)
+{
+    if (count == 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e != hipSuccess) return fail(c, OCPX_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(q, 0, count * sizeof(T));
+    if (e != hipSuccess) return fail(c, OCPX_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return OCPX_OK;
+}
+
+int64_t flen(int field, int Nc)
+{
+    const int nc = Nc + 1, np = 3;
+    switch (field) {
+    case OCPX_F_NI: case OCPX_F_LNI: case OCPX_F_VFI: return nc;
+    case OCPX_F_PHASEEXIST: case OCPX_F_S: case OCPX_F_RHO: case OCPX_F_XI: case OCPX_F_MU:
+    case OCPX_F_RHOP: case OCPX_F_XIP: case OCPX_F_MUP: case OCPX_F_KR: case OCPX_F_PC:
+    case OCPX_F_PJ: return np;
+    case OCPX_F_XIJ: case OCPX_F_RHOX: case OCPX_F_XIX: case OCPX_F_MUX: return np * nc;
+    case OCPX_F_DSEC_DPRI: return np * (nc + 1) * (nc + 1);
+    case OCPX_F_DKRDS: case OCPX_F_DPCDS: return np * np;
+    case OCPX_F_SKIP_ZI: return Nc;
+    case OCPX_F_ITERS: return 8;
+    default: return (field >= 0 && field < OCPX_F_NFIELDS) ? 1 : -1;
+    }
+}
+
+int fesz(int field)
+{
+    switch (field) {
+    case OCPX_F_PHASEEXIST: case OCPX_F_FTYPE_IN: case OCPX_F_FTYPE_OUT: case OCPX_F_SKIP_FLAG:
+    case OCPX_F_ITERS: return 4;
+    default: return 8;
+    }
+}
+
+double* D(ocpx_ctx* c, int field) { return (double*)c->f[field].d; }
+
+CellIO cell_io(ocpx_ctx* c)
+{
+    CellIO io{};
+    io.nb = c->nb;
+    io.nc = c->nc;
+    io.P = D(c, OCPX_F_P); io.Ni = D(c, OCPX_F_NI); io.lNi = D(c, OCPX_F_LNI); io.S = D(c, OCPX_F_S);
+    io.xij = D(c, OCPX_F_XIJ); io.rho = D(c, OCPX_F_RHO); io.xi = D(c, OCPX_F_XI); io.mu = D(c, OCPX_F_MU);
+    io.rhoP = D(c, OCPX_F_RHOP); io.xiP = D(c, OCPX_F_XIP); io.muP = D(c, OCPX_F_MUP);
+    io.rhox = D(c, OCPX_F_RHOX); io.xix = D(c, OCPX_F_XIX); io.mux = D(c, OCPX_F_MUX);
+    io.Nt = D(c, OCPX_F_NT); io.vf = D(c, OCPX_F_VF); io.vfP = D(c, OCPX_F_VFP); io.vfi = D(c, OCPX_F_VFI);
+    io.dXsdXp = D(c, OCPX_F_DSEC_DPRI);
+    io.phaseExist = (const uint32_t*)c->f[OCPX_F_PHASEEXIST].d;
+    io.kr = D(c, OCPX_F_KR); io.Pc = D(c, OCPX_F_PC); io.Pj = D(c, OCPX_F_PJ);
+    io.dKrdS = D(c, OCPX_F_DKRDS); io.dPcdS = D(c, OCPX_F_DPCDS);
+    io.poro = D(c, OCPX_F_PORO); io.poroP = D(c, OCPX_F_POROP); io.rockVp = D(c, OCPX_F_ROCKVP);
+    io.v = c->v; io.depth = c->depth; io.poroInit = c->poroInit;
+    return io;
+}
+
+BsrIO bsr_io(ocpx_ctx* c)
+{
+    BsrIO b{};
+    b.rowptr = c->rowptr; b.colind = c->colind; b.tslot = c->tslot; b.nbr_conn = c->nbr_conn;
+    b.val = c->val; b.rhs = c->rhs;
+    return b;
+}
+
+ConnIO conn_io(ocpx_ctx* c)
+{
+    ConnIO n{};
+    n.nconn = c->nconn; n.cb = c->cb; n.ce = c->ce; n.trans = c->trans;
+    return n;
+}
+
+struct TimerScope {
+    ocpx_ctx* c;
+    int which;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimerScope(ocpx_ctx* cc, int w) : c(cc), which(w)
+    {
+        if (!c->timing) return;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, c->st);
+    }
+    ~TimerScope()
+    {
+        if (!a) return;
+        hipEventRecord(b, c->st);
+        c->tacc[which].pending.push_back({a, b});
+    }
+};
+
+int launch_pe(ocpx_ctx* c, int mode)
+{
+    PeIO io{};
+    io.nb = c->nb;
+    io.P = D(c, OCPX_F_P); io.T = D(c, OCPX_F_T); io.Ni = D(c, OCPX_F_NI);
+    io.phaseExist = (const uint32_t*)c->f[OCPX_F_PHASEEXIST].d;
+    io.S = D(c, OCPX_F_S); io.xij = D(c, OCPX_F_XIJ);
+    io.ftype_in = c->ftype_override ? (const int32_t*)c->f[OCPX_F_FTYPE_IN].d : nullptr;
+    io.skip_flag = (const uint32_t*)c->f[OCPX_F_SKIP_FLAG].d;
+    io.skip_minEig = D(c, OCPX_F_SKIP_MINEIG); io.skip_P = D(c, OCPX_F_SKIP_P);
+    io.skip_T = D(c, OCPX_F_SKIP_T); io.skip_zi = D(c, OCPX_F_SKIP_ZI);
+    io.pe_NP = c->pe_NP; io.pe_nu = c->pe_nu; io.pe_x = c->pe_x;
+    io.ftype_out = (int32_t*)c->f[OCPX_F_FTYPE_OUT].d;
+    io.iters = (int32_t*)c->f[OCPX_F_ITERS].d;
+    io.queue = c->queue;
+    if (mode == OCPX_FLASH_BATCHED) HIPCHK(c, run_fill_u32(c->queue, 0u, 1, c->st));
+    TimerScope t(c, 0);
+    HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
+    return OCPX_OK;
+}
+
+int launch_tail(ocpx_ctx* c)
+{
+    TailIO io{};
+    io.nb = c->nb;
+    io.P = D(c, OCPX_F_P); io.T = D(c, OCPX_F_T); io.Ni = D(c, OCPX_F_NI);
+    io.pe_NP = c->pe_NP; io.pe_nu = c->pe_nu; io.pe_x = c->pe_x;
+    io.ftype_out = (const int32_t*)c->f[OCPX_F_FTYPE_OUT].d;
+    io.phaseExist = (uint32_t*)c->f[OCPX_F_PHASEEXIST].d;
+    io.S = D(c, OCPX_F_S); io.xij = D(c, OCPX_F_XIJ); io.rho = D(c, OCPX_F_RHO); io.xi = D(c, OCPX_F_XI);
+    io.mu = D(c, OCPX_F_MU); io.rhoP = D(c, OCPX_F_RHOP); io.xiP = D(c, OCPX_F_XIP); io.muP = D(c, OCPX_F_MUP);
+    io.rhox = D(c, OCPX_F_RHOX); io.xix = D(c, OCPX_F_XIX); io.mux = D(c, OCPX_F_MUX);
+    io.Nt = D(c, OCPX_F_NT); io.vf = D(c, OCPX_F_VF); io.vfP = D(c, OCPX_F_VFP); io.vfi = D(c, OCPX_F_VFI);
+    io.dXsdXp = D(c, OCPX_F_DSEC_DPRI);
+    io.skip_flag = (uint32_t*)c->f[OCPX_F_SKIP_FLAG].d;
+    io.skip_minEig = D(c, OCPX_F_SKIP_MINEIG); io.skip_P = D(c, OCPX_F_SKIP_P);
+    io.skip_T = D(c, OCPX_F_SKIP_T); io.skip_zi = D(c, OCPX_F_SKIP_ZI);
+    TimerScope t(c, 1);
+    HIPCHK(c, run_tail(c->NC, c->df, io, c->st));
+    return OCPX_OK;
+}
+
+int launch_krpc_rock(ocpx_ctx* c)
+{
+    TimerScope t(c, 2);
+    HIPCHK(c, run_krpc_rock(c->ds, c->rock.Pref, c->rock.cp1, c->rock.cp2, cell_io(c), c->st));
+    return OCPX_OK;
+}
+
+int launch_res_asm(ocpx_ctx* c, double dt, bool dr, bool dj)
+{
+    TimerScope t(c, dj ? 4 : 3);
+    HIPCHK(c, run_res_asm(c->nc, cell_io(c), conn_io(c), bsr_io(c), dt, dr, dj, c->resAbs, c->bmax, c->barg,
+                          c->st));
+    if (dr) HIPCHK(c, run_norms_final(res_asm_blocks(c->nc, c->nb), c->bmax, c->barg, c->norms4, c->st));
+    return OCPX_OK;
+}
+
+int destroy_graph(ocpx_ctx* c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->gmode = -1;
+    return OCPX_OK;
+}
+
+// Device fluid constants from the ABI description (EoS_PR ctor
+// src/OCPEoS.cpp:19-65, ViscosityMethod03 ctor src/OCPFuncPVT.cpp:415-459,
+// OCPPhaseEquilibrium::Setup src/OCPPhaseEquilibrium.cpp:45-64)
+void build_dev_fluid(const ocpx_fluid* fl, DevFluid& h)
+{
+    h = DevFluid{};
+    const double R = R_GAS;
+    
+    h.NC = fl->nc;
+    h.lbc_exact = fl->lbc_exact;
+    h.skip = fl->skipPSA;
+    h.RR_maxIt = fl->RR_maxIt; h.SSMsta_maxIt = fl->SSMsta_maxIt; h.NRsta_maxIt = fl->NRsta_maxIt;
+    h.SSMsp_maxIt = fl->SSMsp_maxIt; h.NRsp_maxIt = fl->NRsp_maxIt;
+    h.RR_tol = fl->RR_tol; h.SSMsta_tol2 = fl->SSMsta_tol * fl->SSMsta_tol; h.eYt = fl->SSMsta_eYt;
+    h.NRsta_tol = fl->NRsta_tol; h.SSMsp_tol2 = fl->SSMsp_tol * fl->SSMsp_tol; h.NRsp_tol = fl->NRsp_tol;
+    for (int i = 0; i < fl->nc; i++) {
+        const double w = fl->acf[i];
+        h.Tc[i] = fl->Tc[i]; h.Pc[i] = fl->Pc[i]; h.logPc[i] = std::log(fl->Pc[i]); h.MW[i] = fl->MW[i];
+        h.Vc[i] = fl->Vc[i]; h.Vcvis[i] = fl->Vcvis[i]; h.OA[i] = fl->OmegaA[i]; h.OB[i] = fl->OmegaB[i];
+        h.vsh[i] = fl->Vshift[i] * (R * fl->OmegaB[i] * fl->Tc[i] / fl->Pc[i]);
+        h.mw[i] = w <= 0.49 ? 0.37464 + 1.54226 * w - 0.26992 * w * w
+                            : 0.379642 + 1.48503 * w - 0.164423 * w * w + 0.016667 * w * w * w;
+        h.kwA[i] = 5.373 * (1 + w);
+        h.auxB[i] = 5.4402 * std::pow(fl->Tc[i], 1.0 / 6) / std::pow(fl->Pc[i], 2.0 / 3);
+        h.sqrtMW[i] = std::sqrt(fl->MW[i]);
+        for (int k = 0; k < fl->nc; k++) h.omk[i * OCPX_NCMAX + k] = 1 - fl->BIC[i * OCPX_MAX_NC + k];
+    }
+    for (int k = 0; k < 5; k++) { h.lbc[k] = fl->LBCcoef[k] * 10; h.pvtw[k] = fl->PVTW[k]; }
+    h.rhoW = fl->rhoW_std;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ocpx_field_len(int field, int32_t Nc) { return flen(field, Nc); }
+
+int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* rock, const ocpx_grid* g,
+                    void* stream, ocpx_ctx** out)
+{
+    if (!out) return OCPX_E_ARG;
+    *out = nullptr;
+    if (!fl || !sat || !rock || !g) return OCPX_E_ARG;
+    if (fl->nc < 2 || fl->nc > 12) return OCPX_E_UNSUPPORTED;
+    if (g->nb <= 0 || g->nb > INT32_MAX || g->nconn < 0 || g->nconn > INT32_MAX) return OCPX_E_ARG;
+    if (sat->nswof < 1 || sat->nswof > OCPX_MAX_TABROWS || sat->nsgof < 1 || sat->nsgof > OCPX_MAX_TABROWS)
+        return OCPX_E_ARG;
+    ocpx_ctx* c = new ocpx_ctx();
+    c->NC = fl->nc;
+    c->nc = fl->nc + 1;
+    c->nb = g->nb;
+    c->nconn = g->nconn;
+    c->rock = *rock;
+    if (stream) c->st = (hipStream_t)stream;
+    else {
+        if (hipStreamCreate(&c->st) != hipSuccess) { delete c; return OCPX_E_HIP; }
+        c->own_stream = true;
+    }
+    // connections must be valid and ordered by bId (iteratorConn order)
+    for (int64_t k = 0; k < g->nconn; k++) {
+        const int32_t b = g->conn_b[k], e = g->conn_e[k];
+        if (b < 0 || e < 0 || b >= g->nb || e >= g->nb || b >= e || (k > 0 && b < g->conn_b[k - 1])) {
+            c->err = "connections must satisfy 0 <= bId < eId < nb and be sorted by bId";
+            *out = c;
+            return OCPX_E_ARG;
+        }
+    }
+    // --- fluid constants (EoS_PR ctor, ViscosityMethod03 ctor, flash controls)
+    build_dev_fluid(fl, c->hf);
+    DevSat hs{};
+    hs.nswof = sat->nswof; hs.nsgof = sat->nsgof;
+    std::memcpy(hs.swof, sat->swof, sizeof(hs.swof));
+    std::memcpy(hs.sgof, sat->sgof, sizeof(hs.sgof));
+
+    int rc;
+#define ALLOC(p, n) if ((rc = dalloc(c, &(p), (n))) != OCPX_OK) { *out = c; return rc; }
+    ALLOC(c->df, 1);
+    ALLOC(c->ds, 1);
+    if (hipMemcpy(c->df, &c->hf, sizeof(c->hf), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->ds, &hs, sizeof(hs), hipMemcpyHostToDevice) != hipSuccess) {
+        *out = c;
+        return fail(c, OCPX_E_HIP, "copy of fluid constants failed");
+    }
+    for (int fi = 0; fi < OCPX_F_NFIELDS; fi++) {
+        c->f[fi].len = (int)flen(fi, c->NC);
+        c->f[fi].esz = fesz(fi);
+        if (c->f[fi].esz == 8) { double* p; ALLOC(p, (size_t)c->f[fi].len * c->nb); c->f[fi].d = p; }
+        else { int32_t* p; ALLOC(p, (size_t)c->f[fi].len * c->nb); c->f[fi].d = p; }
+    }
+    ALLOC(c->v, c->nb); ALLOC(c->depth, c->nb); ALLOC(c->poroInit, c->nb);
+    ALLOC(c->cb, c->nconn); ALLOC(c->ce, c->nconn); ALLOC(c->trans, c->nconn);
+    ALLOC(c->pe_NP, c->nb); ALLOC(c->pe_nu, c->nb); ALLOC(c->pe_x, (size_t)2 * c->NC * c->nb);
+    ALLOC(c->queue, 4);
+    // BSR in OCPMatrix push order: diagonal first, then per connection in
+    // index order, row e gets (e, b) and row b gets (b, e)
+    c->nnzb = c->nb + 2 * c->nconn;
+    std::vector<int64_t> rp(c->nb + 1, 0);
+    std::vector<int64_t> cnt(c->nb, 1);
+    for (int64_t k = 0; k < c->nconn; k++) { cnt[g->conn_b[k]]++; cnt[g->conn_e[k]]++; }
+    for (int64_t n = 0; n < c->nb; n++) rp[n + 1] = rp[n] + cnt[n];
+    std::vector<int32_t> col(c->nnzb), ts(c->nnzb), nbc(c->nnzb);
+    std::vector<int64_t> fill(c->nb);
+    for (int64_t n = 0; n < c->nb; n++) {
+        col[rp[n]] = (int32_t)n; ts[rp[n]] = (int32_t)rp[n]; nbc[rp[n]] = -1; fill[n] = rp[n] + 1;
+    }
+    for (int64_t k = 0; k < c->nconn; k++) {
+        const int32_t b = g->conn_b[k], e = g->conn_e[k];
+        const int64_t se = fill[e]++, sb = fill[b]++;
+        col[se] = b; nbc[se] = (int32_t)k;
+        col[sb] = e; nbc[sb] = (int32_t)k;
+        ts[se] = (int32_t)sb; ts[sb] = (int32_t)se;
+    }
+    const int B = c->nc + 1;
+    ALLOC(c->rowptr, c->nb + 1); ALLOC(c->colind, c->nnzb); ALLOC(c->tslot, c->nnzb); ALLOC(c->nbr_conn, c->nnzb);
+    ALLOC(c->val, (size_t)c->nnzb * B * B); ALLOC(c->rhs, (size_t)c->nb * B); ALLOC(c->resAbs, (size_t)c->nb * B);
+    const int64_t nblk = res_asm_blocks(c->nc, c->nb);
+    ALLOC(c->bmax, 2 * nblk); ALLOC(c->barg, 2 * nblk); ALLOC(c->norms4, 4);
+    ALLOC(c->lam, (size_t)c->nb * B); ALLOC(c->out, (size_t)c->nb * B);
+    ALLOC(c->gT, c->nconn); ALLOC(c->gV, c->nb);
+#undef ALLOC
+    bool okc = hipMemcpy(c->rowptr, rp.data(), rp.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(c->colind, col.data(), col.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(c->tslot, ts.data(), ts.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(c->nbr_conn, nbc.data(), nbc.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(c->v, g->v, c->nb * 8, hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(c->depth, g->depth, c->nb * 8, hipMemcpyHostToDevice) == hipSuccess &&
+               hipMemcpy(c->poroInit, g->poroInit, c->nb * 8, hipMemcpyHostToDevice) == hipSuccess;
+    if (okc && c->nconn > 0)
+        okc = hipMemcpy(c->cb, g->conn_b, c->nconn * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->ce, g->conn_e, c->nconn * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(c->trans, g->trans, c->nconn * 8, hipMemcpyHostToDevice) == hipSuccess;
+    if (!okc) { *out = c; return fail(c, OCPX_E_HIP, "upload of grid data failed"); }
+    // FTYPE_IN defaults to -1 (SkipPSA decides)
+    {
+        std::vector<int32_t> m1(c->nb, -1);
+        if (hipMemcpy(c->f[OCPX_F_FTYPE_IN].d, m1.data(), c->nb * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            *out = c;
+            return fail(c, OCPX_E_HIP, "init of FTYPE_IN failed");
+        }
+    }
+    int dev = 0, ncu = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int per_cu = pe_batched_blocks_per_cu(c->NC);
+    c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
+    *out = c;
+    return OCPX_OK;
+}
+
+void ocpx_ctx_destroy(ocpx_ctx* c)
+{
+    if (!c) return;
+    hipStreamSynchronize(c->st);
+    destroy_graph(c);
+    for (auto& t : c->tacc)
+        for (auto& p : t.pending) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+    for (void* p : c->allocs) hipFree(p);
+    if (c->own_stream) hipStreamDestroy(c->st);
+    delete c;
+}
+
+const char* ocpx_last_error(const ocpx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ocpx_sync(ocpx_ctx* c)
+{
+    if (!c) return OCPX_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OCPX_OK;
+}
+
+int ocpx_set_field(ocpx_ctx* c, int field, const void* host)
+{
+    if (!c || !host || field < 0 || field >= OCPX_F_NFIELDS) return fail(c, OCPX_E_ARG, "bad field");
+    const Field& F = c->f[field];
+    const size_t n = c->nb, L = F.len;
+    std::vector<char> tmp(n * L * F.esz);
+    const char* h = (const char*)host;
+    for (size_t cell = 0; cell < n; cell++)
+        for (size_t k = 0; k < L; k++) std::memcpy(&tmp[(k * n + cell) * F.esz], h + (cell * L + k) * F.esz, F.esz);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(F.d, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    if (field == OCPX_F_FTYPE_IN) {
+        const int32_t* fi = (const int32_t*)host;
+        bool any = false;
+        for (size_t i = 0; i < n; i++) any = any || fi[i] >= 0;
+        c->ftype_override = any;
+    }
+    return OCPX_OK;
+}
+
+int ocpx_get_field(ocpx_ctx* c, int field, void* host)
+{
+    if (!c || !host || field < 0 || field >= OCPX_F_NFIELDS) return fail(c, OCPX_E_ARG, "bad field");
+    const Field& F = c->f[field];
+    const size_t n = c->nb, L = F.len;
+    std::vector<char> tmp(n * L * F.esz);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(tmp.data(), F.d, tmp.size(), hipMemcpyDeviceToHost));
+    char* h = (char*)host;
+    for (size_t cell = 0; cell < n; cell++)
+        for (size_t k = 0; k < L; k++) std::memcpy(h + (cell * L + k) * F.esz, &tmp[(k * n + cell) * F.esz], F.esz);
+    return OCPX_OK;
+}
+
+int ocpx_field_device_ptr(ocpx_ctx* c, int field, void** dptr)
+{
+    if (!c || !dptr || field < 0 || field >= OCPX_F_NFIELDS) return fail(c, OCPX_E_ARG, "bad field");
+    *dptr = c->f[field].d;
+    return OCPX_OK;
+}
+
+int ocpx_phase_equilibrium(ocpx_ctx* c, int mode)
+{
+    if (!c || (mode != OCPX_FLASH_PLAIN && mode != OCPX_FLASH_BATCHED)) return fail(c, OCPX_E_ARG, "bad mode");
+    return launch_pe(c, mode);
+}
+
+int ocpx_get_pe_result(ocpx_ctx* c, int32_t* NP, double* nu, double* x)
+{
+    if (!c) return OCPX_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const size_t n = c->nb;
+    if (NP) HIPCHK(c, hipMemcpy(NP, c->pe_NP, n * 4, hipMemcpyDeviceToHost));
+    std::vector<double> a(n), b(2 * c->NC * n);
+    HIPCHK(c, hipMemcpy(a.data(), c->pe_nu, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(b.data(), c->pe_x, b.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<int32_t> np_(n);
+    HIPCHK(c, hipMemcpy(np_.data(), c->pe_NP, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        if (nu) { nu[2 * i] = a[i]; nu[2 * i + 1] = np_[i] > 1 ? 1 - a[i] : 0.0; }
+        if (x)
+            for (int j = 0; j < 2; j++)
+                for (int k = 0; k < c->NC; k++) x[(i * 2 + j) * c->NC + k] = b[(size_t)(j * c->NC + k) * n + i];
+    }
+    return OCPX_OK;
+}
+
+int ocpx_flash_der(ocpx_ctx* c, int mode)
+{
+    if (!c || (mode != OCPX_FLASH_PLAIN && mode != OCPX_FLASH_BATCHED)) return fail(c, OCPX_E_ARG, "bad mode");
+    int rc = launch_pe(c, mode);
+    if (rc) return rc;
+    rc = launch_tail(c);
+    if (rc) return rc;
+    c->flashed = true;
+    return OCPX_OK;
+}
+
+int ocpx_krpc_der(ocpx_ctx* c)
+{
+    if (!c) return OCPX_E_ARG;
+    return launch_krpc_rock(c);   // rock is cheap and fused; CalRock is idempotent
+}
+
+int ocpx_cal_rock(ocpx_ctx* c)
+{
+    if (!c) return OCPX_E_ARG;
+    return launch_krpc_rock(c);
+}
+
+int ocpx_cal_res(ocpx_ctx* c, double dt, double* resAbs, ocpx_resnorms* norms)
+{
+    if (!c) return OCPX_E_ARG;
+    int rc = launch_res_asm(c, dt, true, false);
+    if (rc) return rc;
+    if (resAbs || norms) HIPCHK(c, hipStreamSynchronize(c->st));
+    if (resAbs) HIPCHK(c, hipMemcpy(resAbs, c->resAbs, (size_t)c->nb * (c->nc + 1) * 8, hipMemcpyDeviceToHost));
+    if (norms) {
+        double h[4];
+        HIPCHK(c, hipMemcpy(h, c->norms4, sizeof(h), hipMemcpyDeviceToHost));
+        norms->maxRelRes_V = h[0]; norms->maxId_V = (int64_t)h[1];
+        norms->maxRelRes_N = h[2]; norms->maxId_N = (int64_t)h[3];
+    }
+    return OCPX_OK;
+}
+
+int ocpx_assemble(ocpx_ctx* c, double dt)
+{
+    if (!c) return OCPX_E_ARG;
+    if (!c->flashed) return fail(c, OCPX_E_STATE, "ocpx_assemble before ocpx_flash_der");
+    int rc = launch_res_asm(c, dt, true, true);
+    if (rc) return rc;
+    c->assembled = true;
+    return OCPX_OK;
+}
+
+int ocpx_get_bsr(ocpx_ctx* c, int64_t* nnzb, int64_t* rowptr, int32_t* colind, double* val, double* rhs)
+{
+    if (!c) return OCPX_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const int B = c->nc + 1;
+    if (nnzb) *nnzb = c->nnzb;
+    if (rowptr) HIPCHK(c, hipMemcpy(rowptr, c->rowptr, (c->nb + 1) * 8, hipMemcpyDeviceToHost));
+    if (colind) HIPCHK(c, hipMemcpy(colind, c->colind, c->nnzb * 4, hipMemcpyDeviceToHost));
+    if (val) HIPCHK(c, hipMemcpy(val, c->val, (size_t)c->nnzb * B * B * 8, hipMemcpyDeviceToHost));
+    if (rhs) HIPCHK(c, hipMemcpy(rhs, c->rhs, (size_t)c->nb * B * 8, hipMemcpyDeviceToHost));
+    return OCPX_OK;
+}
+
+int ocpx_bsr_device_ptrs(ocpx_ctx* c, void** rowptr, void** colind, void** val, void** rhs)
+{
+    if (!c) return OCPX_E_ARG;
+    if (rowptr) *rowptr = c->rowptr;
+    if (colind) *colind = c->colind;
+    if (val) *val = c->val;
+    if (rhs) *rhs = c->rhs;
+    return OCPX_OK;
+}
+
+int ocpx_get_solution(ocpx_ctx* c, const double* u, double dSmax)
+{
+    if (!c || !u) return OCPX_E_ARG;
+    if (!c->flashed) return fail(c, OCPX_E_STATE, "ocpx_get_solution needs dXsdXp from ocpx_flash_der");
+    const size_t B = c->nc + 1;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(c->lam, u, c->nb * B * 8, hipMemcpyHostToDevice));
+    HIPCHK(c, run_get_solution(c->nc, D(c, OCPX_F_P), D(c, OCPX_F_NI), D(c, OCPX_F_S), D(c, OCPX_F_XIJ),
+                               D(c, OCPX_F_DSEC_DPRI), c->lam, dSmax, c->nb, c->st));
+    return OCPX_OK;
+}
+
+int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
+{
+    if (!c || (mode != OCPX_FLASH_PLAIN && mode != OCPX_FLASH_BATCHED)) return fail(c, OCPX_E_ARG, "bad mode");
+    if (!use_graph) {
+        int rc = ocpx_flash_der(c, mode);
+        if (rc) return rc;
+        rc = launch_krpc_rock(c);
+        if (rc) return rc;
+        rc = launch_res_asm(c, dt, true, true);
+        if (rc) return rc;
+        c->assembled = true;
+        return OCPX_OK;
+    }
+    if (c->own_stream == false && c->st == nullptr) return fail(c, OCPX_E_STATE, "graph capture needs a stream");
+    if (!c->gexec || c->gdt != dt || c->gmode != mode) {
+        destroy_graph(c);
+        const bool tm = c->timing;
+        c->timing = false;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        int rc = launch_pe(c, mode);
+        if (!rc) rc = launch_tail(c);
+        if (!rc) rc = launch_krpc_rock(c);
+        if (!rc) rc = launch_res_asm(c, dt, true, true);
+        hipGraph_t gph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->st, &gph);
+        c->timing = tm;
+        if (rc) { if (gph) hipGraphDestroy(gph); return rc; }
+        if (e != hipSuccess) return fail(c, OCPX_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+        c->graph = gph;
+        HIPCHK(c, hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+        c->gdt = dt;
+        c->gmode = mode;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+    c->flashed = c->assembled = true;
+    return OCPX_OK;
+}
+
+int ocpx_jt_lambda_dev(ocpx_ctx* c, const double* lam, double* out)
+{
+    if (!c || !lam || !out) return OCPX_E_ARG;
+    if (!c->assembled) return fail(c, OCPX_E_STATE, "ocpx_jt_lambda before ocpx_assemble");
+    HIPCHK(c, run_jt_lambda(c->nc, c->nb, bsr_io(c), lam, out, c->st));
+    return OCPX_OK;
+}
+
+int ocpx_jt_lambda(ocpx_ctx* c, const double* lam, double* out)
+{
+    if (!c || !lam || !out) return OCPX_E_ARG;
+    const size_t B = c->nc + 1;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(c->lam, lam, c->nb * B * 8, hipMemcpyHostToDevice));
+    int rc = ocpx_jt_lambda_dev(c, c->lam, c->out);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(out, c->out, c->nb * B * 8, hipMemcpyDeviceToHost));
+    return OCPX_OK;
+}
+
+int ocpx_dR_dparams_dev(ocpx_ctx* c, double dt, const double* lam, double* dT, double* dVp)
+{
+    if (!c || !lam) return OCPX_E_ARG;
+    if (!c->flashed) return fail(c, OCPX_E_STATE, "ocpx_dR_dparams before ocpx_flash_der");
+    HIPCHK(c, run_dR_dtrans(c->nc, cell_io(c), conn_io(c), dt, lam, dT ? dT : c->gT, dVp ? dVp : c->gV, c->st));
+    return OCPX_OK;
+}
+
+int ocpx_dR_dparams(ocpx_ctx* c, double dt, const double* lam, double* dT, double* dVp)
+{
+    if (!c || !lam) return OCPX_E_ARG;
+    const size_t B = c->nc + 1;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(c->lam, lam, c->nb * B * 8, hipMemcpyHostToDevice));
+    int rc = ocpx_dR_dparams_dev(c, dt, c->lam, c->gT, c->gV);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (dT && c->nconn) HIPCHK(c, hipMemcpy(dT, c->gT, c->nconn * 8, hipMemcpyDeviceToHost));
+    if (dVp) HIPCHK(c, hipMemcpy(dVp, c->gV, c->nb * 8, hipMemcpyDeviceToHost));
+    return OCPX_OK;
+}
+
+int ocpx_timing_enable(ocpx_ctx* c, int on)
+{
+    if (!c) return OCPX_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (auto& t : c->tacc) {
+        for (auto& p : t.pending) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+        t.pending.clear();
+        t.total_ms = 0;
+        t.calls = 0;
+    }
+    c->timing = on != 0;
+    return OCPX_OK;
+}
+
+int ocpx_timing_get(ocpx_ctx* c, int which, double* avg_ms, int64_t* calls)
+{
+    if (!c || which < 0 || which > 4) return OCPX_E_ARG;
+    TimingAcc& t = c->tacc[which];
+    for (auto& p : t.pending) {
+        HIPCHK(c, hipEventSynchronize(p.second));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, p.first, p.second));
+        t.total_ms += ms;
+        t.calls++;
+        hipEventDestroy(p.first);
+        hipEventDestroy(p.second);
+    }
+    t.pending.clear();
+    if (avg_ms) *avg_ms = t.calls ? t.total_ms / t.calls : 0.0;
+    if (calls) *calls = t.calls;
+    return OCPX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// ocpx_kernels.hpp — kernel argument blocks and launchers shared by the
+// HIP translation units and the host code (ocpx_host.cpp).
+//
+// Every per-cell array is component-major structure-of-arrays:
+// field[k * nb + cell].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ocpx {
+
+struct DevFluid;
+struct DevSat;
+
+// phase-equilibrium kernel (pe.hip)
+struct PeIO {
+    int64_t nb;
+    const double* P;            // [nb]
+    const double* T;            // [nb] degC
+    const double* Ni;           // [nc][nb]
+    const uint32_t* phaseExist; // [3][nb]  previous step
+    const double* S;            // [3][nb]  previous step
+    const double* xij;          // [3*nc][nb] previous step
+    const int32_t* ftype_in;    // [nb] or null; < 0 -> SkipPSA
+    const uint32_t* skip_flag;  // SkipPSAVarset
+    const double* skip_minEig;
+    const double* skip_P;
+    const double* skip_T;
+    const double* skip_zi;      // [Nc][nb]
+    int32_t* pe_NP;             // [nb]
+    double* pe_nu;              // [nb] nu of PE phase 0
+    double* pe_x;               // [2*Nc][nb]
+    int32_t* ftype_out;         // [nb]
+    int32_t* iters;             // [8][nb] or null
+    unsigned* queue;            // work-queue counter (batched mode)
+};
+
+// flash-derivative tail (tail.hip): properties, labels, dVf, dXsdXp,
+// SkipPSA next-step data and the PassFlashValue writes
+struct TailIO {
+    int64_t nb;
+    const double* P;
+    const double* T;
+    const double* Ni;
+    const int32_t* pe_NP;
+    const double* pe_nu;
+    const double* pe_x;
+    const int32_t* ftype_out;
+    // outputs (BulkVarSet mirror)
+    uint32_t* phaseExist;
+    double *S, *xij, *rho, *xi, *mu, *rhoP, *xiP, *muP, *rhox, *xix, *mux;
+    double *Nt, *vf, *vfP, *vfi, *dXsdXp;
+    uint32_t* skip_flag;
+    double *skip_minEig, *skip_P, *skip_T, *skip_zi;
+};
+
+hipError_t run_pe(int NC, const DevFluid* f, const PeIO& io, int mode, int grid_waves,
+                  hipStream_t st);
+hipError_t run_tail(int NC, const DevFluid* f, const TailIO& io, hipStream_t st);
+
+// FIM kernels (fim.hip)
+struct CellIO {
+    int64_t nb;
+    int32_t nc;                 // bulk components (Nc + 1)
+    const double *P, *Ni, *lNi, *S, *xij, *rho, *xi, *mu, *rhoP, *xiP, *muP;
+    const double *rhox, *xix, *mux, *Nt, *vf, *vfP, *vfi, *dXsdXp;
+    const uint32_t* phaseExist;
+    double *kr, *Pc, *Pj, *dKrdS, *dPcdS, *poro, *poroP, *rockVp;
+    const double *v, *depth, *poroInit;
+};
+
+// block-CSR and cell adjacency
+struct BsrIO {
+    const int64_t* rowptr;      // [nb+1]
+    const int32_t* colind;      // [nnzb]
+    const int32_t* tslot;       // [nnzb] slot of the transposed block
+    const int32_t* nbr_conn;    // [nnzb] connection of the slot (-1 = diagonal)
+    double* val;                // [nnzb][b][b]
+    double* rhs;                // [nb][b]
+};
+
+struct ConnIO {
+    int64_t nconn;
+    const int32_t *cb, *ce;
+    const double* trans;
+};
+
+hipError_t run_krpc_rock(const DevSat* s, double rPref, double rcp1, double rcp2, const CellIO& c,
+                         hipStream_t st);
+hipError_t run_res_asm(int nc, const CellIO& c, const ConnIO& cn, const BsrIO& bsr, double dt,
+                       bool do_res, bool do_jac, double* res, double* blockmax, int64_t* blockarg,
+                       hipStream_t st);
+hipError_t run_norms_final(int64_t nblocks, const double* blockmax, const int64_t* blockarg,
+                           double* out4, hipStream_t st);
+hipError_t run_get_solution(int nc, double* P, double* Ni, double* S, double* xij,
+                            const double* dXsdXp, const double* u, double dSmax, int64_t nb,
+                            hipStream_t st);
+hipError_t run_jt_lambda(int nc, int64_t nb, const BsrIO& bsr, const double* lambda, double* out,
+                         hipStream_t st);
+hipError_t run_dR_dtrans(int nc, const CellIO& c, const ConnIO& cn, double dt,
+                         const double* lambda, double* dTrans, double* dVp, hipStream_t st);
+hipError_t run_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st);
+
+}  // namespace ocpx

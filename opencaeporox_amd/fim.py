@@ -1,0 +1,169 @@
+"""Python face of the C ABI, shaped like the reference's IsoT_FIM loops.
+
+`IsoTFIM` owns one ocpx context (device-resident BulkVarSet mirror + BSR)
+and exposes the per-Newton-step entry points under the reference's names:
+
+    CalFlash        IsoT_FIM::CalFlash + PassFlashValue  (src/IsoThermalMethod.cpp:1016-1071)
+    CalKrPc         IsoT_FIM::CalKrPc                     (:1073-1088)
+    CalRock         IsothermalMethod::CalRock             (:19-30)
+    CalRes          IsoT_FIM::CalRes                      (:1090-1185, no wells)
+    AssembleMatBulks IsoT_FIM::AssembleMatBulks + CopyRhs (:1187-1269, :742-752)
+    GetSolution     IsoT_FIM::GetSolution                 (:1282-1418, bulk part)
+
+Everything runs through libocpx.so on the GPU; there is no CPU path here.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+PLAIN, BATCHED = 0, 1
+
+
+class OcpxError(RuntimeError):
+    pass
+
+
+class IsoTFIM:
+    def __init__(self, fluid, sat, rock, grid, stream=None):
+        self.lib = abi.load()
+        self.fluid, self.sat, self.rock, self.grid = fluid, sat, rock, grid
+        self.Nc = fluid.nc
+        self.nc = self.Nc + 1
+        self.bs = self.nc + 1
+        self.nb = grid.nb
+        self.nconn = grid.nconn
+        self._g = grid.to_abi()
+        h = C.c_void_p()
+        rc = self.lib.ocpx_ctx_create(C.byref(fluid), C.byref(sat), C.byref(rock), C.byref(self._g),
+                                      stream, C.byref(h))
+        self.h = h
+        if rc != 0:
+            msg = self.lib.ocpx_last_error(h).decode() if h.value else ""
+            if h.value:
+                self.lib.ocpx_ctx_destroy(h)
+                self.h = C.c_void_p()
+            raise OcpxError(f"ocpx_ctx_create failed ({rc}): {msg}")
+
+    @classmethod
+    def from_workload(cls, wl, stream=None):
+        obj = cls(wl.fluid, wl.sat, wl.rock, wl.grid, stream)
+        obj.set_state(wl.state)
+        return obj
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.ocpx_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise OcpxError(f"{what} failed ({rc}): {self.lib.ocpx_last_error(self.h).decode()}")
+
+    # ---- state exchange (BulkVarSet layout)
+    def set(self, name, arr):
+        fid = abi.FIELD_ID[name]
+        dt = abi.INT_FIELDS.get(name, np.float64)
+        a = np.ascontiguousarray(np.asarray(arr, dtype=dt).ravel())
+        n = self.nb * abi.field_len(name, self.Nc)
+        if a.size != n:
+            raise ValueError(f"{name}: expected {n} entries, got {a.size}")
+        self._chk(self.lib.ocpx_set_field(self.h, fid, a.ctypes.data), f"set {name}")
+
+    def get(self, name):
+        fid = abi.FIELD_ID[name]
+        dt = abi.INT_FIELDS.get(name, np.float64)
+        a = np.zeros(self.nb * abi.field_len(name, self.Nc), dt)
+        self._chk(self.lib.ocpx_get_field(self.h, fid, a.ctypes.data), f"get {name}")
+        return a
+
+    def set_state(self, st):
+        for k, v in st.items():
+            self.set(k, v)
+
+    def sync(self):
+        self._chk(self.lib.ocpx_sync(self.h), "sync")
+
+    # ---- hot path
+    def PhaseEquilibrium(self, mode=PLAIN):
+        self._chk(self.lib.ocpx_phase_equilibrium(self.h, mode), "phase equilibrium")
+
+    def pe_result(self):
+        NP = np.zeros(self.nb, np.int32)
+        nu = np.zeros(2 * self.nb)
+        x = np.zeros(2 * self.nb * self.Nc)
+        self._chk(self.lib.ocpx_get_pe_result(self.h, NP.ctypes.data, nu.ctypes.data, x.ctypes.data),
+                  "pe result")
+        return NP, nu.reshape(-1, 2), x.reshape(-1, 2, self.Nc)
+
+    def CalFlash(self, mode=PLAIN):
+        self._chk(self.lib.ocpx_flash_der(self.h, mode), "flash")
+
+    def CalKrPc(self):
+        self._chk(self.lib.ocpx_krpc_der(self.h), "krpc")
+
+    def CalRock(self):
+        self._chk(self.lib.ocpx_cal_rock(self.h), "rock")
+
+    def CalRes(self, dt, fetch=True):
+        res = np.zeros(self.nb * self.bs) if fetch else None
+        norms = abi.ResNorms()
+        self._chk(self.lib.ocpx_cal_res(self.h, dt, res.ctypes.data if fetch else None,
+                                        C.byref(norms) if fetch else None), "residual")
+        if not fetch:
+            return None
+        return res, np.array([norms.maxRelRes_V, norms.maxId_V, norms.maxRelRes_N, norms.maxId_N])
+
+    def AssembleMatBulks(self, dt):
+        self._chk(self.lib.ocpx_assemble(self.h, dt), "assemble")
+
+    def bsr(self):
+        nnz = C.c_int64()
+        self._chk(self.lib.ocpx_get_bsr(self.h, C.byref(nnz), None, None, None, None), "bsr size")
+        n = nnz.value
+        rowptr = np.zeros(self.nb + 1, np.int64)
+        col = np.zeros(n, np.int32)
+        val = np.zeros(n * self.bs * self.bs)
+        rhs = np.zeros(self.nb * self.bs)
+        self._chk(self.lib.ocpx_get_bsr(self.h, C.byref(nnz), rowptr.ctypes.data, col.ctypes.data,
+                                        val.ctypes.data, rhs.ctypes.data), "bsr")
+        return rowptr, col, val.reshape(n, self.bs, self.bs), rhs
+
+    def GetSolution(self, u, dSmax=0.1):
+        u = np.ascontiguousarray(u, np.float64)
+        self._chk(self.lib.ocpx_get_solution(self.h, u.ctypes.data, dSmax), "get solution")
+
+    def newton_pass(self, dt, mode=BATCHED, graph=False):
+        self._chk(self.lib.ocpx_newton_pass(self.h, dt, mode, 1 if graph else 0), "newton pass")
+
+    # ---- differentiation
+    def jt_lambda(self, lam):
+        lam = np.ascontiguousarray(lam, np.float64)
+        out = np.zeros_like(lam)
+        self._chk(self.lib.ocpx_jt_lambda(self.h, lam.ctypes.data, out.ctypes.data), "J^T lambda")
+        return out
+
+    def dR_dparams(self, dt, lam):
+        lam = np.ascontiguousarray(lam, np.float64)
+        dT = np.zeros(max(self.nconn, 1))
+        dV = np.zeros(self.nb)
+        self._chk(self.lib.ocpx_dR_dparams(self.h, dt, lam.ctypes.data, dT.ctypes.data, dV.ctypes.data),
+                  "dR/dparams")
+        return dT[: self.nconn], dV
+
+    # ---- timing
+    def timing(self, on=True):
+        self._chk(self.lib.ocpx_timing_enable(self.h, 1 if on else 0), "timing")
+
+    def timing_get(self, which):
+        ms = C.c_double()
+        calls = C.c_int64()
+        self._chk(self.lib.ocpx_timing_get(self.h, which, C.byref(ms), C.byref(calls)), "timing get")
+        return ms.value, calls.value

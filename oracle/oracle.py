@@ -1,0 +1,315 @@
+"""ctypes wrapper of the C oracle (build/libocpx_oracle.so) and of the
+reference harness (_ref/libocpref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "build", "libocpx_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libocpref.so")
+
+D = np.float64
+
+
+def build(ref=True):
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if ref and os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class OrcBulk(C.Structure):
+    _fields_ = [("nb", C.c_int64)] + [(n, C.c_void_p) for n in (
+        "P", "T", "Ni", "lNi", "S", "xij", "rho", "xi", "mu", "rhoP", "xiP", "muP",
+        "rhox", "xix", "mux", "Nt", "vf", "vfP", "vfi", "dSec_dPri", "kr", "Pc", "Pj",
+        "dKrdS", "dPcdS", "poro", "poroP", "rockVp", "v", "poroInit", "depth",
+        "phaseExist", "ftype_in", "ftype_out", "skip_flag", "skip_minEig", "skip_P",
+        "skip_T", "skip_zi", "iters", "pe_ill")]
+
+
+# mapping OrcBulk member -> (field name of opencaeporox_amd.abi, dtype)
+_MEMBERS = {
+    "P": "P", "T": "T", "Ni": "NI", "lNi": "LNI", "S": "S", "xij": "XIJ", "rho": "RHO",
+    "xi": "XI", "mu": "MU", "rhoP": "RHOP", "xiP": "XIP", "muP": "MUP", "rhox": "RHOX",
+    "xix": "XIX", "mux": "MUX", "Nt": "NT", "vf": "VF", "vfP": "VFP", "vfi": "VFI",
+    "dSec_dPri": "DSEC_DPRI", "kr": "KR", "Pc": "PC", "Pj": "PJ", "dKrdS": "DKRDS",
+    "dPcdS": "DPCDS", "poro": "PORO", "poroP": "POROP", "rockVp": "ROCKVP",
+    "phaseExist": "PHASEEXIST", "ftype_in": "FTYPE_IN", "ftype_out": "FTYPE_OUT",
+    "skip_flag": "SKIP_FLAG", "skip_minEig": "SKIP_MINEIG", "skip_P": "SKIP_P",
+    "skip_T": "SKIP_T", "skip_zi": "SKIP_ZI", "iters": "ITERS",
+}
+INT_FIELDS = {"PHASEEXIST": np.uint32, "FTYPE_IN": np.int32, "FTYPE_OUT": np.int32,
+              "SKIP_FLAG": np.uint32, "ITERS": np.int32}
+
+
+def field_len(name, Nc):
+    nc, np_ = Nc + 1, 3
+    table = {"NI": nc, "LNI": nc, "PHASEEXIST": np_, "S": np_, "XIJ": np_ * nc,
+             "RHO": np_, "XI": np_, "MU": np_, "RHOP": np_, "XIP": np_, "MUP": np_,
+             "RHOX": np_ * nc, "XIX": np_ * nc, "MUX": np_ * nc, "VFI": nc,
+             "DSEC_DPRI": np_ * (nc + 1) * (nc + 1), "KR": np_, "PC": np_, "PJ": np_,
+             "DKRDS": np_ * np_, "DPCDS": np_ * np_, "SKIP_ZI": Nc, "ITERS": 8}
+    return table.get(name, 1)
+
+
+class OracleState:
+    """All BulkVarSet fields as numpy arrays (zero-initialised like the
+    reference's vector::resize) plus the static cell data."""
+
+    def __init__(self, Nc, nb, v, poroInit, depth, init=None):
+        self.Nc, self.nb = Nc, nb
+        self.a = {}
+        for m, f in _MEMBERS.items():
+            dt = INT_FIELDS.get(f, D)
+            self.a[f] = np.zeros(nb * field_len(f, Nc), dt)
+        self.a["FTYPE_IN"][:] = -1
+        self.ill = np.zeros(nb, np.int32)
+        self.v = np.ascontiguousarray(v, D)
+        self.poroInit = np.ascontiguousarray(poroInit, D)
+        self.depth = np.ascontiguousarray(depth, D)
+        if init:
+            for k, val in init.items():
+                self.a[k][:] = np.asarray(val).ravel()
+
+    def struct(self):
+        b = OrcBulk()
+        b.nb = self.nb
+        for m, f in _MEMBERS.items():
+            setattr(b, m, self.a[f].ctypes.data)
+        b.v, b.poroInit, b.depth = (self.v.ctypes.data, self.poroInit.ctypes.data,
+                                    self.depth.ctypes.data)
+        b.pe_ill = self.ill.ctypes.data
+        return b
+
+    def copy(self):
+        o = OracleState.__new__(OracleState)
+        o.Nc, o.nb, o.v, o.poroInit, o.depth = self.Nc, self.nb, self.v, self.poroInit, self.depth
+        o.a = {k: v.copy() for k, v in self.a.items()}
+        o.ill = self.ill.copy()
+        return o
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            build(ref=False)
+        self.lib = L = C.CDLL(path)
+        vp, dbl, i64 = C.c_void_p, C.c_double, C.c_int64
+        L.orc_cubic_root.restype = C.c_int
+        L.orc_cubic_root.argtypes = [dbl, dbl, dbl, vp]
+        for n in ("orc_eos_fug",):
+            getattr(L, n).argtypes = [vp, dbl, dbl, vp, vp, vp, vp]
+        for n in ("orc_eos_lnfugx", "orc_eos_lnfugp"):
+            getattr(L, n).argtypes = [vp, dbl, dbl, vp, vp]
+        for n in ("orc_eos_lnfugn", "orc_eos_lnphin"):
+            getattr(L, n).argtypes = [vp, dbl, dbl, vp, dbl, vp]
+        L.orc_eos_vmder.restype = dbl
+        L.orc_eos_vmder.argtypes = [vp, dbl, dbl, vp, vp, vp]
+        L.orc_phase_equilibrium.argtypes = [vp, i64] + [vp] * 12
+        L.orc_lbc.restype = dbl
+        L.orc_lbc.argtypes = [vp, dbl, vp, dbl, dbl, vp, vp, vp]
+        L.orc_pvtw.argtypes = [vp, dbl, vp]
+        L.orc_cal_flash.argtypes = [vp, vp]
+        L.orc_cal_krpc.argtypes = [vp, vp]
+        L.orc_kr_pc.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_cal_rock.argtypes = [vp, vp]
+        L.orc_cal_res.argtypes = [vp, vp, i64, vp, vp, vp, dbl, vp, vp]
+        L.orc_assemble.argtypes = [vp, vp, i64, vp, vp, vp, dbl, vp, vp, vp]
+        L.orc_get_solution.argtypes = [vp, vp, vp, dbl]
+        L.orc_cartesian_conns.restype = i64
+        L.orc_cartesian_conns.argtypes = [C.c_int] * 3 + [vp] * 10
+
+    # ---- single-composition probes
+    def cubic(self, a, b, c):
+        z = np.zeros(3)
+        k = self.lib.orc_cubic_root(a, b, c, _p(z))
+        return k, z
+
+    def fug(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D)
+        fug, phi, abz = np.zeros(len(x)), np.zeros(len(x)), np.zeros(3)
+        self.lib.orc_eos_fug(C.byref(fl), P, T, _p(x), _p(fug), _p(phi), _p(abz))
+        return fug, phi, abz
+
+    def lnfugx(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x) ** 2)
+        self.lib.orc_eos_lnfugx(C.byref(fl), P, T, _p(x), _p(o)); return o.reshape(len(x), -1)
+
+    def lnfugn(self, fl, P, T, x, nt):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x) ** 2)
+        self.lib.orc_eos_lnfugn(C.byref(fl), P, T, _p(x), nt, _p(o)); return o.reshape(len(x), -1)
+
+    def lnphin(self, fl, P, T, x, nt):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x) ** 2)
+        self.lib.orc_eos_lnphin(C.byref(fl), P, T, _p(x), nt, _p(o)); return o.reshape(len(x), -1)
+
+    def lnfugp(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x))
+        self.lib.orc_eos_lnfugp(C.byref(fl), P, T, _p(x), _p(o)); return o
+
+    def vmder(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D); vmx = np.zeros(len(x)); vmP = np.zeros(1)
+        v = self.lib.orc_eos_vmder(C.byref(fl), P, T, _p(x), _p(vmP), _p(vmx))
+        return v, vmP[0], vmx
+
+    def lbc(self, fl, T, x, xi, xiP, xix):
+        x = np.ascontiguousarray(x, D); xix = np.ascontiguousarray(xix, D)
+        mux = np.zeros(len(x)); muP = np.zeros(1)
+        mu = self.lib.orc_lbc(C.byref(fl), T, _p(x), xi, xiP, _p(xix), _p(muP), _p(mux))
+        return mu, muP[0], mux
+
+    def pvtw(self, fl, P):
+        o = np.zeros(6); self.lib.orc_pvtw(C.byref(fl), P, _p(o)); return o
+
+    def kr_pc(self, sat, S):
+        S = np.ascontiguousarray(S, D)
+        kr, pc, dk, dp = np.zeros(3), np.zeros(3), np.zeros(9), np.zeros(9)
+        self.lib.orc_kr_pc(C.byref(sat), _p(S), _p(kr), _p(pc), _p(dk), _p(dp))
+        return kr, pc, dk, dp
+
+    def phase_equilibrium(self, fl, P, T, z, ftype=None, lNP=None, lx=None):
+        n, Nc = z.shape
+        P, T, z = (np.ascontiguousarray(a, D) for a in (P, T, z))
+        NP = np.zeros(n, np.int32); nu = np.zeros(2 * n); x = np.zeros(2 * n * Nc)
+        fo = np.zeros(n, np.int32); it = np.zeros(8 * n, np.int32); ill = np.zeros(n, np.int32)
+        ft = None if ftype is None else np.ascontiguousarray(ftype, np.int32)
+        ln = None if lNP is None else np.ascontiguousarray(lNP, np.int32)
+        lxx = None if lx is None else np.ascontiguousarray(lx, D)
+        self.lib.orc_phase_equilibrium(C.byref(fl), n, _p(P), _p(T), _p(z), _p(ft), _p(ln),
+                                       _p(lxx), _p(NP), _p(nu), _p(x), _p(fo), _p(it), _p(ill))
+        self.last_ill = ill
+        return NP, nu.reshape(n, 2), x.reshape(n, 2, Nc), fo, it.reshape(n, 8)
+
+    # ---- FIM loops on an OracleState
+    def cal_flash(self, fl, st):
+        b = st.struct(); self.lib.orc_cal_flash(C.byref(fl), C.byref(b))
+
+    def cal_krpc(self, sat, st):
+        b = st.struct(); self.lib.orc_cal_krpc(C.byref(sat), C.byref(b))
+
+    def cal_rock(self, rock, st):
+        b = st.struct(); self.lib.orc_cal_rock(C.byref(rock), C.byref(b))
+
+    def cal_res(self, fl, st, grid, dt):
+        b = st.struct()
+        len_ = fl.nc + 2
+        res = np.zeros(st.nb * len_); norms = np.zeros(4)
+        cb, ce, tr = (np.ascontiguousarray(grid.conn_b, np.int32),
+                      np.ascontiguousarray(grid.conn_e, np.int32), np.ascontiguousarray(grid.trans, D))
+        self.lib.orc_cal_res(C.byref(fl), C.byref(b), grid.nconn, _p(cb), _p(ce), _p(tr), dt,
+                             _p(res), _p(norms))
+        return res, norms
+
+    def assemble(self, fl, st, grid, dt):
+        b = st.struct()
+        bs = fl.nc + 2
+        nnz = st.nb + 2 * grid.nconn
+        rowptr = np.zeros(st.nb + 1, np.int64); col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz * bs * bs)
+        cb, ce, tr = (np.ascontiguousarray(grid.conn_b, np.int32),
+                      np.ascontiguousarray(grid.conn_e, np.int32), np.ascontiguousarray(grid.trans, D))
+        self.lib.orc_assemble(C.byref(fl), C.byref(b), grid.nconn, _p(cb), _p(ce), _p(tr), dt,
+                              _p(rowptr), _p(col), _p(val))
+        return rowptr, col, val.reshape(nnz, bs, bs)
+
+    def get_solution(self, fl, st, u, dSmax):
+        b = st.struct(); u = np.ascontiguousarray(u, D)
+        self.lib.orc_get_solution(C.byref(fl), C.byref(b), _p(u), dSmax)
+
+    def newton_pass(self, wl, st, dt):
+        """flash + krpc + rock + res + assemble, as IsoT_FIM::UpdateProperty
+        followed by AssembleMat (src/IsoThermalMethod.cpp:811-836, 742-752)."""
+        self.cal_flash(wl.fluid, st)
+        self.cal_krpc(wl.sat, st)
+        self.cal_rock(wl.rock, st)
+        res, norms = self.cal_res(wl.fluid, st, wl.grid, dt)
+        A = self.assemble(wl.fluid, st, wl.grid, dt)
+        return res, norms, A
+
+
+def state_from_workload(wl):
+    return OracleState(wl.fluid.nc, wl.grid.nb, wl.grid.v, wl.grid.poro, wl.grid.depth,
+                       init=wl.state)
+
+
+class Ref:
+    """The reference's own EoS / PVT / SAT / rock sources (oracle/_ref)."""
+
+    def __init__(self, path=REF_SO):
+        self.lib = L = C.CDLL(path)
+        vp, dbl = C.c_void_p, C.c_double
+        L.ref_cubic.restype = C.c_int
+        L.ref_cubic.argtypes = [dbl, dbl, dbl, vp]
+        L.ref_eos_fug.argtypes = [vp, dbl, dbl, vp, vp, vp, vp]
+        for n in ("ref_eos_lnfugx", "ref_eos_lnfugp"):
+            getattr(L, n).argtypes = [vp, dbl, dbl, vp, vp]
+        for n in ("ref_eos_lnfugn", "ref_eos_lnphin"):
+            getattr(L, n).argtypes = [vp, dbl, dbl, vp, dbl, vp]
+        L.ref_eos_vmder.restype = dbl
+        L.ref_eos_vmder.argtypes = [vp, dbl, dbl, vp, vp, vp]
+        L.ref_pvtw.argtypes = [vp, dbl, vp]
+        L.ref_lbc.restype = dbl
+        L.ref_lbc.argtypes = [vp, dbl, vp, dbl, dbl, vp, vp, vp]
+        L.ref_kr_pc.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.ref_rock.argtypes = [vp, dbl, dbl, vp]
+
+    @staticmethod
+    def available():
+        return os.path.exists(REF_SO)
+
+    def cubic(self, a, b, c):
+        z = np.zeros(3); k = self.lib.ref_cubic(a, b, c, _p(z)); return k, z
+
+    def fug(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D)
+        fug, phi, abz = np.zeros(len(x)), np.zeros(len(x)), np.zeros(3)
+        self.lib.ref_eos_fug(C.byref(fl), P, T, _p(x), _p(fug), _p(phi), _p(abz))
+        return fug, phi, abz
+
+    def lnfugx(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x) ** 2)
+        self.lib.ref_eos_lnfugx(C.byref(fl), P, T, _p(x), _p(o)); return o.reshape(len(x), -1)
+
+    def lnfugn(self, fl, P, T, x, nt):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x) ** 2)
+        self.lib.ref_eos_lnfugn(C.byref(fl), P, T, _p(x), nt, _p(o)); return o.reshape(len(x), -1)
+
+    def lnphin(self, fl, P, T, x, nt):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x) ** 2)
+        self.lib.ref_eos_lnphin(C.byref(fl), P, T, _p(x), nt, _p(o)); return o.reshape(len(x), -1)
+
+    def lnfugp(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D); o = np.zeros(len(x))
+        self.lib.ref_eos_lnfugp(C.byref(fl), P, T, _p(x), _p(o)); return o
+
+    def vmder(self, fl, P, T, x):
+        x = np.ascontiguousarray(x, D); vmx = np.zeros(len(x)); vmP = np.zeros(1)
+        v = self.lib.ref_eos_vmder(C.byref(fl), P, T, _p(x), _p(vmP), _p(vmx))
+        return v, vmP[0], vmx
+
+    def pvtw(self, fl, P):
+        o = np.zeros(6); self.lib.ref_pvtw(C.byref(fl), P, _p(o)); return o
+
+    def lbc(self, fl, T, x, xi, xiP, xix):
+        x = np.ascontiguousarray(x, D); xix = np.ascontiguousarray(xix, D)
+        mux = np.zeros(len(x)); muP = np.zeros(1)
+        mu = self.lib.ref_lbc(C.byref(fl), T, _p(x), xi, xiP, _p(xix), _p(muP), _p(mux))
+        return mu, muP[0], mux
+
+    def kr_pc(self, sat, S):
+        S = np.ascontiguousarray(S, D)
+        kr, pc, dk, dp = np.zeros(3), np.zeros(3), np.zeros(9), np.zeros(9)
+        self.lib.ref_kr_pc(C.byref(sat), _p(S), _p(kr), _p(pc), _p(dk), _p(dp))
+        return kr, pc, dk, dp
+
+    def rock(self, rk, P, poroInit):
+        o = np.zeros(2); self.lib.ref_rock(C.byref(rk), P, poroInit, _p(o)); return o
