@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Flash + FIM-assembly throughput on one MI355X.
+
+Workload (per step, FP64): 1,048,576 independent cells of a 9-component
+Peng-Robinson mixture plus water at 344 K, compositions Dirichlet(0.5), 30%
+of the pressures within +-5% of the cell's saturation pressure (found by
+bisection on the device's own phase count), the rest U(10, 28) MPa.  One
+step = ResetToLastTimeStep + flash with dXsdXp + kr/Pc + rock + residual +
+Jacobian assembly over every cell, i.e. one IsoT_FIM Newton pass
+(src/IsoThermalMethod.cpp:811-836, 1016-1269) restarted from the same state
+so every step does identical work.  Inputs are resident in HBM before the
+timed region.  value = cells/s of the persistent work-queue ("batched")
+flash path; the one-cell-per-lane ("plain") path is timed beside it.
+
+Also reported: the 100x100x50 (500,000 cells, Nc = 6) assembly pass with its
+HBM roofline fraction, and the C oracle timed on the host on a sample of the
+same cells (cpu_baseline).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "flash+FIM-assembly cells/s (FP64, 1 GPU) batched vs plain; % FP64/HBM roofline"
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--cells", type=int, default=1 << 20)
+    p.add_argument("--Nc", type=int, default=9)
+    p.add_argument("--seed", type=int, default=2)
+    p.add_argument("--cpu-cells", type=int, default=32768,
+                   help="oracle sample size for cpu_baseline (0 = skip)")
+    p.add_argument("--assembly", type=str, default="100x100x50",
+                   help="grid of the assembly measurement, '' to skip")
+    p.add_argument("--modes", type=str, default="plain,batched")
+    return p.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flash_workload(n, Nc, seed):
+    """The 1M-cell flash workload; saturation pressures from the device."""
+    from opencaeporox_amd import workload as W
+    from opencaeporox_amd.fim import BATCHED, IsoTFIM
+    rng = np.random.Generator(np.random.PCG64(seed))
+    wl, z = W.flash_cells(n, Nc, seed=seed + 1000, alpha=0.5)
+    near = rng.uniform(0, 1, n) < 0.3
+    idx = np.where(near)[0]
+    # bisection on the phase count of the near cells (device phase equilibrium)
+    sub, _ = W.flash_cells(idx.size, Nc, seed=seed + 1000, alpha=0.5)
+    sub.state["NI"] = wl.state["NI"].reshape(n, -1)[idx].ravel()
+    sub.state["P"] = wl.state["P"][idx]
+    dev = IsoTFIM.from_workload(sub)
+
+    def nphase(P, zz, T):
+        dev.set("P", P)
+        dev.PhaseEquilibrium(BATCHED)
+        return dev.pe_result()[0]
+
+    t = time.perf_counter()
+    psat = W.saturation_pressure(nphase, z[idx], np.full(idx.size, 344.0))
+    dev.close()
+    log(f"saturation pressures of {idx.size} cells in {time.perf_counter() - t:.1f} s "
+        f"({np.isnan(psat).sum()} without a two-phase region)")
+    P = rng.uniform(100.0, 280.0, n)
+    ok = np.isfinite(psat)
+    P[idx[ok]] = psat[ok] * (1 + rng.uniform(-0.05, 0.05, ok.sum()))
+    st = W.initial_state(wl.fluid, wl.rock, wl.grid, P, z, rng, perturb=False)
+    wl.state = st
+    wl.name = (f"{n} independent cells, Nc={Nc} PR + H2O, T=344 K, z~Dirichlet(0.5), "
+               f"{ok.sum()} cells within 5% of P_sat")
+    return wl, int(ok.sum())
+
+
+def time_steps(dev, mode, dt, steps, warmup, sync):
+    for _ in range(warmup):
+        dev.ResetToLastTimeStep()
+        dev.newton_pass(dt, mode)
+    sync()
+    dev.timing(True)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.ResetToLastTimeStep()
+        dev.newton_pass(dt, mode)
+    sync()
+    t1 = time.perf_counter()
+    k = {name: dev.timing_get(i)[0] for i, name in enumerate(("pe", "tail", "krpc_rock", "res", "asm"))}
+    dev.timing(False)
+    return (t1 - t0) * 1e3 / steps, k
+
+
+def cpu_baseline(wl, ncell, dt):
+    """The oracle (the C restatement of the reference loops, one host core)
+    on the first ncell cells of the same workload: flash + kr/Pc + rock +
+    residual + assembly."""
+    from opencaeporox_amd import workload as W
+    from oracle.oracle import Oracle, OracleState
+    o = Oracle()
+    nb = wl.grid.nb
+    g = wl.grid
+    sub = W.Grid(ncell, 1, 1, g.v[:ncell], g.depth[:ncell], g.poro[:ncell],
+                 np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    init = {k: np.asarray(v).reshape(nb, -1)[:ncell].ravel() for k, v in wl.state.items()}
+    st = OracleState(wl.fluid.nc, ncell, sub.v, sub.poro, sub.depth, init=init)
+    t = time.perf_counter()
+    o.cal_flash(wl.fluid, st)
+    o.cal_krpc(wl.sat, st)
+    o.cal_rock(wl.rock, st)
+    o.cal_res(wl.fluid, st, sub, dt)
+    o.assemble(wl.fluid, st, sub, dt)
+    sec = time.perf_counter() - t
+    return {"value": ncell / sec, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"first {ncell} cells of the same workload, one full pass, {sec:.1f} s"}
+
+
+def assembly_measure(spec, steps, warmup, sync, dt):
+    from opencaeporox_amd import workload as W
+    from opencaeporox_amd.fim import BATCHED, IsoTFIM
+    from opencaeporox_amd.roofline import PEAK_HBM_GBS, assembly_bytes
+    nx, ny, nz = (int(v) for v in spec.split("x"))
+    wl = W.config3(nx, ny, nz, Nc=6)
+    dev = IsoTFIM.from_workload(wl)
+    dev.UpdateLastTimeStep()
+    ms, k = time_steps(dev, BATCHED, dt, steps, warmup, sync)
+    nb, nconn, nc = dev.nb, dev.nconn, dev.nc
+    byts = assembly_bytes(nb, nconn, nc)
+    gbs = byts / (k["asm"] * 1e-3) / 1e9
+    dev.close()
+    return {"workload": wl.name, "cells": nb, "conns": nconn, "block": nc + 1,
+            "ms_per_pass": ms, "cells_per_s": nb / (ms * 1e-3), "kernel_ms": k,
+            "assembly_conns_per_s": nconn / (k["asm"] * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": gbs / PEAK_HBM_GBS, "traffic": None,
+                         "algorithmic_bytes": byts}}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(cells are independent: a multi-GPU run would split them into contiguous blocks)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    from opencaeporox_amd.fim import BATCHED, PLAIN, IsoTFIM
+    from opencaeporox_amd.roofline import PEAK_FP64_TFLOPS, flash_flops
+
+    def sync():
+        torch.cuda.synchronize()
+
+    dt = 1.0
+    t = time.perf_counter()
+    wl, n_near = flash_workload(a.cells, a.Nc, a.seed)
+    log(f"workload ready in {time.perf_counter() - t:.1f} s: {wl.name}")
+    dev = IsoTFIM.from_workload(wl)
+    dev.UpdateLastTimeStep()
+    out = {}
+    modes = [m for m in a.modes.split(",") if m]
+    for name in modes:
+        mode = PLAIN if name == "plain" else BATCHED
+        ms, k = time_steps(dev, mode, dt, a.steps, a.warmup, sync)
+        it = dev.get("ITERS")
+        out[name] = {"ms_per_step": ms, "cells_per_s": dev.nb / (ms * 1e-3), "kernel_ms": k,
+                     "flops": flash_flops(it, a.Nc)}
+        log(f"{name}: {ms:.2f} ms/step, {dev.nb / (ms * 1e-3):.3e} cells/s, kernels {k}")
+    head = out.get("batched", out[modes[-1]])
+    tflops = head["flops"] / (head["kernel_ms"]["pe"] * 1e-3) / 1e12
+    line = {
+        "metric": METRIC, "value": head["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "cells": dev.nb, "Nc": a.Nc, "near_saturation_cells": n_near,
+                   "pass": "reset + flash_der + kr/Pc + rock + residual + assembly",
+                   "flash": "batched" if "batched" in out else modes[-1]},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tflops / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "pe_batched" if "batched" in out else "pe_plain",
+                     "flops_per_launch": head["flops"], "kernel_ms": head["kernel_ms"]["pe"]},
+    }
+    if "plain" in out and "batched" in out:
+        line["plain"] = {"value": out["plain"]["cells_per_s"], "ms_per_step": out["plain"]["ms_per_step"],
+                         "kernel_ms": out["plain"]["kernel_ms"]}
+        line["batched_over_plain"] = out["batched"]["cells_per_s"] / out["plain"]["cells_per_s"]
+        line["kernel_ms"] = out["batched"]["kernel_ms"]
+    dev.close()
+    if a.assembly:
+        line["assembly"] = assembly_measure(a.assembly, max(3, a.steps // 2), 1, sync, dt)
+        log(f"assembly: {line['assembly']['kernel_ms']}")
+    if a.cpu_cells > 0:
+        line["cpu_baseline"] = cpu_baseline(wl, min(a.cpu_cells, wl.grid.nb), dt)
+        log(f"cpu baseline: {line['cpu_baseline']}")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

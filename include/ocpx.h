@@ -231,6 +231,18 @@ int  ocpx_bsr_device_ptrs(ocpx_ctx* ctx, void** rowptr, void** colind,
  * [nb*(nc+1)]. */
 int  ocpx_get_solution(ocpx_ctx* ctx, const double* u, double dSmax);
 
+/* ≙ IsoT_FIM::UpdateLastTimeStep (src/IsoThermalMethod.cpp:1473-1515) with
+ * SkipPSAVarset::UpdateLastTimeStep (src/AcceleratePEC.cpp:24-31): keeps a
+ * device copy of the bulk state (rock, Nt, vf, P, Pj, Pc, phaseExist, S,
+ * xij, rho, xi, mu, kr, their derivatives, dSec_dPri, the SkipPSA data) and
+ * sets lNi (OCPX_F_LNI) = Ni.  Device-to-device, on the context stream. */
+int  ocpx_update_last_step(ocpx_ctx* ctx);
+/* ≙ IsoT_FIM::ResetToLastTimeStep (:1420-1464) with
+ * SkipPSAVarset::ResetToLastTimeStep (src/AcceleratePEC.cpp:15-22):
+ * restores that copy and Ni = lNi (no wells, no residual recomputation).
+ * OCPX_E_STATE before the first ocpx_update_last_step. */
+int  ocpx_reset_to_last_step(ocpx_ctx* ctx);
+
 /* One full Newton-step pass: flash_der + krpc + rock + res + assemble.
  * With use_graph != 0 the pass is captured once into a hipGraph and
  * replayed afterwards (dt is baked in at capture; re-captured if dt or

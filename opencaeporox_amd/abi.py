@@ -112,6 +112,8 @@ def load(path=None):
                                            C.POINTER(vp)]),
         "ocpx_get_solution": (C.c_int, [vp, vp, dbl]),
         "ocpx_newton_pass": (C.c_int, [vp, dbl, C.c_int, C.c_int]),
+        "ocpx_update_last_step": (C.c_int, [vp]),
+        "ocpx_reset_to_last_step": (C.c_int, [vp]),
         "ocpx_jt_lambda": (C.c_int, [vp, vp, vp]),
         "ocpx_dR_dparams": (C.c_int, [vp, dbl, vp, vp, vp]),
         "ocpx_jt_lambda_dev": (C.c_int, [vp, vp, vp]),

@@ -27,8 +27,7 @@ def hipcc():
 
 def flags():
     return ["-x", "hip", f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
-            # packed FP32 instructions stay off (they have given wrong results
-            # on these machines); this code is FP64 anyway
+            # packed FP32 instructions are kept off; this code is FP64 anyway
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", "-I", os.path.join(PKG, "..", "include")]
 

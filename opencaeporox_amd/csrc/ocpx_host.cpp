@@ -50,6 +50,8 @@ struct ocpx_ctx {
     DevSat* ds = nullptr;
     ocpx_rock rock{};
     Field f[OCPX_F_NFIELDS];
+    Field lf[OCPX_F_NFIELDS];   // last-time-step copies (UpdateLastTimeStep)
+    bool have_last = false;
     bool ftype_override = false;
     // geometry
     double *v = nullptr, *depth = nullptr, *poroInit = nullptr, *trans = nullptr;
@@ -135,6 +137,28 @@ int fesz(int field)
 }
 
 double* D(ocpx_ctx* c, int field) { return (double*)c->f[field].d; }
+
+// BulkVarSet layout (cell-major, [n][L]) <-> device layout (component-major,
+// [L][n]), in tiles of 64 cells so both sides stream.
+template <class T>
+void to_soa(const T* aos, T* soa, size_t n, size_t L)
+{
+    for (size_t c0 = 0; c0 < n; c0 += 64) {
+        const size_t c1 = c0 + 64 < n ? c0 + 64 : n;
+        for (size_t k = 0; k < L; k++)
+            for (size_t c = c0; c < c1; c++) soa[k * n + c] = aos[c * L + k];
+    }
+}
+
+template <class T>
+void to_aos(const T* soa, T* aos, size_t n, size_t L)
+{
+    for (size_t c0 = 0; c0 < n; c0 += 64) {
+        const size_t c1 = c0 + 64 < n ? c0 + 64 : n;
+        for (size_t c = c0; c < c1; c++)
+            for (size_t k = 0; k < L; k++) aos[c * L + k] = soa[k * n + c];
+    }
+}
 
 CellIO cell_io(ocpx_ctx* c)
 {
@@ -433,11 +457,11 @@ int ocpx_set_field(ocpx_ctx* c, int field, const void* host)
     const Field& F = c->f[field];
     const size_t n = c->nb, L = F.len;
     std::vector<char> tmp(n * L * F.esz);
-    const char* h = (const char*)host;
-    for (size_t cell = 0; cell < n; cell++)
-        for (size_t k = 0; k < L; k++) std::memcpy(&tmp[(k * n + cell) * F.esz], h + (cell * L + k) * F.esz, F.esz);
+    if (F.esz == 8) to_soa((const uint64_t*)host, (uint64_t*)tmp.data(), n, L);
+    else to_soa((const uint32_t*)host, (uint32_t*)tmp.data(), n, L);
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipMemcpy(F.d, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    if (field == OCPX_F_DSEC_DPRI) c->flashed = true;   // dXsdXp supplied by the caller
     if (field == OCPX_F_FTYPE_IN) {
         const int32_t* fi = (const int32_t*)host;
         bool any = false;
@@ -455,9 +479,8 @@ int ocpx_get_field(ocpx_ctx* c, int field, void* host)
     std::vector<char> tmp(n * L * F.esz);
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipMemcpy(tmp.data(), F.d, tmp.size(), hipMemcpyDeviceToHost));
-    char* h = (char*)host;
-    for (size_t cell = 0; cell < n; cell++)
-        for (size_t k = 0; k < L; k++) std::memcpy(h + (cell * L + k) * F.esz, &tmp[(k * n + cell) * F.esz], F.esz);
+    if (F.esz == 8) to_aos((const uint64_t*)tmp.data(), (uint64_t*)host, n, L);
+    else to_aos((const uint32_t*)tmp.data(), (uint32_t*)host, n, L);
     return OCPX_OK;
 }
 
@@ -575,6 +598,54 @@ int ocpx_get_solution(ocpx_ctx* c, const double* u, double dSmax)
     HIPCHK(c, hipMemcpy(c->lam, u, c->nb * B * 8, hipMemcpyHostToDevice));
     HIPCHK(c, run_get_solution(c->nc, D(c, OCPX_F_P), D(c, OCPX_F_NI), D(c, OCPX_F_S), D(c, OCPX_F_XIJ),
                                D(c, OCPX_F_DSEC_DPRI), c->lam, dSmax, c->nb, c->st));
+    return OCPX_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// The bulk fields IsoT_FIM::UpdateLastTimeStep / ResetToLastTimeStep carry
+// (src/IsoThermalMethod.cpp:1420-1515) plus the SkipPSA variable set
+// (src/AcceleratePEC.cpp:15-31).  Ni's last-step copy is the lNi field itself.
+const int kLastStepFields[] = {
+    OCPX_F_PORO, OCPX_F_POROP, OCPX_F_ROCKVP, OCPX_F_NT, OCPX_F_VF, OCPX_F_P, OCPX_F_PJ,
+    OCPX_F_PC, OCPX_F_PHASEEXIST, OCPX_F_S, OCPX_F_XIJ, OCPX_F_RHO, OCPX_F_XI, OCPX_F_MU,
+    OCPX_F_KR, OCPX_F_VFP, OCPX_F_VFI, OCPX_F_RHOP, OCPX_F_RHOX, OCPX_F_XIP, OCPX_F_XIX,
+    OCPX_F_MUP, OCPX_F_MUX, OCPX_F_DPCDS, OCPX_F_DKRDS, OCPX_F_DSEC_DPRI, OCPX_F_SKIP_FLAG,
+    OCPX_F_SKIP_MINEIG, OCPX_F_SKIP_P, OCPX_F_SKIP_T, OCPX_F_SKIP_ZI};
+
+size_t field_bytes(const ocpx_ctx* c, int fid) { return (size_t)c->nb * c->f[fid].len * c->f[fid].esz; }
+}  // namespace
+
+extern "C" {
+
+int ocpx_update_last_step(ocpx_ctx* c)
+{
+    if (!c) return OCPX_E_ARG;
+    for (int fid : kLastStepFields) {
+        if (!c->lf[fid].d) {
+            char* p = nullptr;
+            int rc = dalloc(c, &p, field_bytes(c, fid));
+            if (rc) return rc;
+            c->lf[fid] = c->f[fid];
+            c->lf[fid].d = p;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->lf[fid].d, c->f[fid].d, field_bytes(c, fid), hipMemcpyDeviceToDevice, c->st));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->f[OCPX_F_LNI].d, c->f[OCPX_F_NI].d, field_bytes(c, OCPX_F_NI),
+                             hipMemcpyDeviceToDevice, c->st));
+    c->have_last = true;
+    return OCPX_OK;
+}
+
+int ocpx_reset_to_last_step(ocpx_ctx* c)
+{
+    if (!c) return OCPX_E_ARG;
+    if (!c->have_last) return fail(c, OCPX_E_STATE, "ocpx_reset_to_last_step before ocpx_update_last_step");
+    for (int fid : kLastStepFields)
+        HIPCHK(c, hipMemcpyAsync(c->f[fid].d, c->lf[fid].d, field_bytes(c, fid), hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->f[OCPX_F_NI].d, c->f[OCPX_F_LNI].d, field_bytes(c, OCPX_F_NI),
+                             hipMemcpyDeviceToDevice, c->st));
     return OCPX_OK;
 }
 

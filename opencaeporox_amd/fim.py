@@ -9,6 +9,7 @@ and exposes the per-Newton-step entry points under the reference's names:
     CalRes          IsoT_FIM::CalRes                      (:1090-1185, no wells)
     AssembleMatBulks IsoT_FIM::AssembleMatBulks + CopyRhs (:1187-1269, :742-752)
     GetSolution     IsoT_FIM::GetSolution                 (:1282-1418, bulk part)
+    UpdateLastTimeStep / ResetToLastTimeStep              (:1420-1515)
 
 Everything runs through libocpx.so on the GPU; there is no CPU path here.
 """
@@ -139,6 +140,12 @@ class IsoTFIM:
     def GetSolution(self, u, dSmax=0.1):
         u = np.ascontiguousarray(u, np.float64)
         self._chk(self.lib.ocpx_get_solution(self.h, u.ctypes.data, dSmax), "get solution")
+
+    def UpdateLastTimeStep(self):
+        self._chk(self.lib.ocpx_update_last_step(self.h), "update last step")
+
+    def ResetToLastTimeStep(self):
+        self._chk(self.lib.ocpx_reset_to_last_step(self.h), "reset to last step")
 
     def newton_pass(self, dt, mode=BATCHED, graph=False):
         self._chk(self.lib.ocpx_newton_pass(self.h, dt, mode, 1 if graph else 0), "newton pass")

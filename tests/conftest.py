@@ -20,6 +20,16 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
+def oracle_fma():
+    """The oracle built with fused multiply-add contraction."""
+    import oracle.oracle as OO
+    path = os.path.join(OO.HERE, "build", "libocpx_oracle_fma.so")
+    if not os.path.exists(path):
+        pytest.skip("oracle FMA build missing")
+    return OO.Oracle(path)
+
+
+@pytest.fixture(scope="session")
 def ref():
     from oracle.oracle import Ref
     if not Ref.available():
@@ -38,3 +48,7 @@ def close(a, b, rtol=1e-10, scale=None):
     bound = rtol * (np.abs(b) + scale)
     ok = (err <= bound) | (a == b)
     return bool(np.all(ok)), float(np.max(err / np.maximum(np.abs(b) + scale, 1e-300))) if b.size else 0.0
+
+
+from oracle.parity import (PARITY_RTOL, SPLIT_ILL, SPREAD_MULT, perturbed_oracle_states,  # noqa: E402,F401
+                           rel_err, rounding_sensitive, rounding_spread)
