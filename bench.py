@@ -136,6 +136,8 @@ def assembly_measure(spec, steps, warmup, sync, dt):
     wl = W.config3(nx, ny, nz, Nc=6)
     dev = IsoTFIM.from_workload(wl)
     dev.UpdateLastTimeStep()
+    dev.autotune_flash()
+    dev.ResetToLastTimeStep()
     ms, k = time_steps(dev, BATCHED, dt, steps, warmup, sync)
     nb, nconn, nc = dev.nb, dev.nconn, dev.nc
     byts = assembly_bytes(nb, nconn, nc)
@@ -174,8 +176,13 @@ def main():
     dev.UpdateLastTimeStep()
     out = {}
     modes = [m for m in a.modes.split(",") if m]
+    tune = None
     for name in modes:
         mode = PLAIN if name == "plain" else BATCHED
+        if mode == BATCHED:
+            dev.ResetToLastTimeStep()
+            tune = dev.autotune_flash()
+            log(f"autotuner (ms per mapping, lanes per cell): {tune} -> {dev.flash_mapping()}")
         ms, k = time_steps(dev, mode, dt, a.steps, a.warmup, sync)
         it = dev.get("ITERS")
         out[name] = {"ms_per_step": ms, "cells_per_s": dev.nb / (ms * 1e-3), "kernel_ms": k,
@@ -190,10 +197,13 @@ def main():
         "data": "synthetic",
         "config": {"workload": wl.name, "cells": dev.nb, "Nc": a.Nc, "near_saturation_cells": n_near,
                    "pass": "reset + flash_der + kr/Pc + rock + residual + assembly",
-                   "flash": "batched" if "batched" in out else modes[-1]},
+                   "flash": "batched" if "batched" in out else modes[-1],
+                   "lanes_per_cell": dev.flash_mapping() if "batched" in out else 1,
+                   "autotune_ms": tune},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "pe_batched" if "batched" in out else "pe_plain",
+                     "kernel": ("pe_team_batched" if dev.flash_mapping() > 1 else "pe_batched")
+                               if "batched" in out else "pe_plain",
                      "flops_per_launch": head["flops"], "kernel_ms": head["kernel_ms"]["pe"]},
     }
     if "plain" in out and "batched" in out:

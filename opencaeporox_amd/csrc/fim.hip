@@ -4,15 +4,25 @@
 // Assembly (IsoT_FIM::CalRes + AssembleMatBulks, src/IsoThermalMethod.cpp:
 // 1090-1269, with OCPConvection01::CalFlux/AssembleMatFIM,
 // src/OCPConvection.cpp:22-187, and BulkAccumuTerm01, src/BulkAccumuModule.cpp:
-// 24-70) is row-owner and atomic-free: each block row of the BSR is produced
-// by b = nc+1 lanes, lane r owning column r of every block of the row.  The
-// row's slots come in OCPMatrix push order (ascending connection index), so
-// the diagonal accumulates its terms in the reference's order.  A flux block
-// dt*(dF/dXp + dF/dXs * dXs/dXp) is never formed as a GEMM: per phase, dF/dXs
-// is an outer product x_up (x) q plus a scaled identity, so a column costs
-// O(np*b) instead of O(np*b^2).  Each connection's two blocks are computed by
-// both rows that need them; the doubled arithmetic is cheaper than the HBM
-// round trip of a connection-ordered scatter.
+// 24-70) runs in two passes, both atomic-free:
+//
+//  1. krpc_aux (one cell per b = nc+1 lanes, lane r = primary variable r):
+//     kr/Pc and rock as in the reference, then, per phase j, the total
+//     derivatives with respect to the primary variables of the three
+//     quantities a connection needs: the mobility lambda_j = xi_j kr_j / mu_j,
+//     the phase pressure P_j = P + Pc_j and the mass density rho_j, chained
+//     through dXs/dXp once per cell.
+//  2. res_asm (one block row per b lanes, lane i = equation i): for each
+//     connection of the row, in OCPMatrix push order, the flux of component i
+//     F_i = T lambda_U x_ijU dPhi_j and its row of dF/dX on either side,
+//         dF_i/dX_s = T dPhi_j [s == U] (x_ij dlambda_U + lambda_U dx_ij,U)
+//                   + T lambda_U x_ij dPhi_j/dX_s,
+//     with dPhi_j/dX_b = dP_jb - g dz w_b drho_jb, dPhi_j/dX_e = -dP_je -
+//     g dz w_e drho_je.  That is the reference's dt (dF/dXp + dF/dXs dXs/dXp)
+//     with the product taken once per cell instead of once per connection.
+//     Each lane writes one contiguous row of each block of its row.  Both
+//     rows of a connection compute its flux; that doubled arithmetic is
+//     cheaper than a connection-ordered scatter.
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 
@@ -43,18 +53,10 @@ OCPX_HD void tab_eval(const double* tab, int nrow, double v, double* out, double
 }
 
 // IsoT_FIM::CalKrPc (:1073-1088) via OCPFlowMethod_OGW01::CalKrPcDer
-// (src/OCPFlowMethod.cpp:63-73) and Stone II (src/OCPFuncSAT.cpp:162-186);
-// IsothermalMethod::CalRock (:19-30) via OCPRockIsoT_Linear (src/OCPRock.cpp:19-24)
-__global__ void __launch_bounds__(256) krpc_rock(const DevSat* __restrict__ sp, double Pref, double cp1,
-                                                 double cp2, CellIO c)
+// (src/OCPFlowMethod.cpp:63-73) and Stone II (src/OCPFuncSAT.cpp:162-186)
+OCPX_HD void krpc_cell(const DevSat& s, double Sg, double Sw, double* kr, double* pc, double* dk, double* dp)
 {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= c.nb) return;
-    const DevSat& s = *sp;
-    const size_t nb = c.nb;
-    const double Sg = c.S[nb + n], Sw = c.S[2 * nb + n];
     double d[4], sl[4];
-    double kr[3], pc[3], dk[9], dp[9];
     UNROLL for (int k = 0; k < 9; k++) { dk[k] = 0; dp[k] = 0; }
     tab_eval(s.swof, s.nswof, Sw, d, sl);
     kr[2] = d[1];
@@ -80,18 +82,69 @@ __global__ void __launch_bounds__(256) krpc_rock(const DevSat* __restrict__ sp, 
         dk[2] = krocw * ((dkrowdSw / krocw + dk[8]) * (krog / krocw + kr[1]) - dk[8]);
         dk[1] = krocw * ((krow / krocw + kr[2]) * (dkrogdSg / krocw + dk[4]) - dk[4]);
     }
+}
+
+// Connection aux record of a cell, per phase j at aux + (n*3 + j)*AUXJ:
+// [dlambda[b], dPj[b], drho[b], lambda], AUXJ = 3b + 1.
+template <int NCB>
+__global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, double Pref, double cp1,
+                                               double cp2, CellIO c)
+{
+    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, AUXJ = 3 * B + 1;
+    const int lane = threadIdx.x, g = lane / B, r = lane % B;
+    const int64_t n = (int64_t)blockIdx.x * G + g;
+    if (g >= G || n >= c.nb) return;
+    const size_t nb = c.nb;
+    double kr[3], pc[3], dk[9], dp[9];
+    krpc_cell(*sp, c.S[nb + n], c.S[2 * nb + n], kr, pc, dk, dp);
     const double P = c.P[n];
-    UNROLL for (int j = 0; j < 3; j++) {
-        c.kr[j * nb + n] = kr[j];
-        c.Pc[j * nb + n] = pc[j];
-        c.Pj[j * nb + n] = P + pc[j];
+    if (r == 0) {
+        UNROLL for (int j = 0; j < 3; j++) {
+            c.kr[j * nb + n] = kr[j];
+            c.Pc[j * nb + n] = pc[j];
+            c.Pj[j * nb + n] = P + pc[j];
+        }
+        UNROLL for (int k = 0; k < 9; k++) { c.dKrdS[k * nb + n] = dk[k]; c.dPcdS[k * nb + n] = dp[k]; }
+        // IsothermalMethod::CalRock (:19-30) via OCPRockIsoT_Linear (src/OCPRock.cpp:19-24)
+        const double dP = P - Pref;
+        const double poro = c.poroInit[n] * (1 + (cp1 + cp2 / 2 * dP) * dP);
+        c.poro[n] = poro;
+        c.poroP[n] = c.poroInit[n] * (cp1 + cp2 * dP);
+        c.rockVp[n] = c.v[n] * poro;
     }
-    UNROLL for (int k = 0; k < 9; k++) { c.dKrdS[k * nb + n] = dk[k]; c.dPcdS[k * nb + n] = dp[k]; }
-    const double dP = P - Pref;
-    const double poro = c.poroInit[n] * (1 + (cp1 + cp2 / 2 * dP) * dP);
-    c.poro[n] = poro;
-    c.poroP[n] = c.poroInit[n] * (cp1 + cp2 * dP);
-    c.rockVp[n] = c.v[n] * poro;
+    if (!c.aux) return;
+    const double* X = c.dXsdXp + (size_t)n * REC;     // row k, column r: X[k * B + r]
+    double dS[np];
+    UNROLL for (int k = 0; k < np; k++) dS[k] = X[k * B + r];
+    double* A = c.aux + (size_t)n * np * AUXJ;
+    const double one = r == 0 ? 1.0 : 0.0;
+    UNROLL for (int j = 0; j < np; j++) {
+        const bool ex = c.phaseExist[j * nb + n] != 0;
+        double drho = 0, dxi = 0, dmu = 0;
+        if (ex) {
+            drho = one * c.rhoP[j * nb + n];
+            dxi = one * c.xiP[j * nb + n];
+            dmu = one * c.muP[j * nb + n];
+            UNROLL for (int m = 0; m < nc; m++) {
+                const double xm = X[(np + j * nc + m) * B + r];
+                drho += c.rhox[(size_t)(j * nc + m) * nb + n] * xm;
+                dxi += c.xix[(size_t)(j * nc + m) * nb + n] * xm;
+                dmu += c.mux[(size_t)(j * nc + m) * nb + n] * xm;
+            }
+        }
+        double dkr = 0, dPj = one;
+        UNROLL for (int k = 0; k < np; k++) { dkr += dk[j * 3 + k] * dS[k]; dPj += dp[j * 3 + k] * dS[k]; }
+        double lam = 0, dlam = 0;
+        if (ex) {
+            const double xi = c.xi[j * nb + n], mu = c.mu[j * nb + n];
+            lam = xi * kr[j] / mu;
+            dlam = kr[j] / mu * dxi + xi / mu * dkr - xi * kr[j] / (mu * mu) * dmu;
+        }
+        A[j * AUXJ + r] = dlam;
+        A[j * AUXJ + B + r] = dPj;
+        A[j * AUXJ + 2 * B + r] = drho;
+        if (r == 0) A[j * AUXJ + 3 * B] = lam;
+    }
 }
 
 // --------------------------------------------------------- residual + Jacobian
@@ -122,115 +175,82 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
                                               int do_jac, double* __restrict__ resAbs,
                                               double* __restrict__ bmax, int64_t* __restrict__ barg)
 {
-    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, XR = np * B;   // XR: dXsdXp rows
+    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, AUXJ = 3 * B + 1;
     const int lane = threadIdx.x;
-    const int r = lane / G, g = lane % G;
+    const int g = lane / B, i = lane % B;              // i: equation (block row)
     const size_t nb = c.nb;
     const int64_t n = (int64_t)blockIdx.x * G + g;
-    const bool valid = (r < B) && (n < c.nb);
+    const bool valid = (g < G) && (n < c.nb);
     __shared__ double sV[64], sN[64];
     __shared__ int64_t iV[64], iN[64];
 
     double res = 0, absV = -1, absN = -1;
     if (valid) {
-        double Xo[XR];                     // column r of own dXsdXp
-        if (do_jac) UNROLL for (int k = 0; k < XR; k++) Xo[k] = c.dXsdXp[(size_t)(k * B + r) * nb + n];
         // accumulation (BulkAccumuTerm01::CalResFIM / CaldFdXpFIM)
         double diag[B];
         if (do_jac) {
-            UNROLL for (int i = 0; i < B; i++) diag[i] = (i == r && i > 0) ? 1.0 : 0.0;
-            diag[0] = r == 0 ? c.v[n] * c.poroP[n] - c.vfP[n] : -c.vfi[(size_t)(r - 1) * nb + n];
+            UNROLL for (int k = 0; k < B; k++) diag[k] = (i > 0 && k == i) ? 1.0 : 0.0;
+            if (i == 0) {
+                diag[0] = c.v[n] * c.poroP[n] - c.vfP[n];
+                UNROLL for (int k = 1; k < B; k++) diag[k] = -c.vfi[(size_t)(k - 1) * nb + n];
+            }
         }
-        if (do_res) res = r == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(r - 1) * nb + n] - c.lNi[(size_t)(r - 1) * nb + n];
+        if (do_res) res = i == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(i - 1) * nb + n] - c.lNi[(size_t)(i - 1) * nb + n];
 
         const int64_t s0 = bsr.rowptr[n], s1 = bsr.rowptr[n + 1];
         for (int64_t s = s0 + 1; s < s1; s++) {
             const int cc = bsr.nbr_conn[s];
             const int64_t bI = cn.cb[cc], eI = cn.ce[cc];
             const bool ownB = (bI == n);
-            const int64_t m = ownB ? eI : bI;
             const double T = cn.trans[cc];
-            double Xm[XR];
-            if (do_jac) UNROLL for (int k = 0; k < XR; k++) Xm[k] = c.dXsdXp[(size_t)(k * B + r) * nb + m];
-            double colB[B], colE[B];
-            UNROLL for (int i = 0; i < B; i++) { colB[i] = 0; colE[i] = 0; }
-            double flux = 0;
             const double dGamma = GRAVF * (c.depth[bI] - c.depth[eI]);
+            double rb[B], re[B];                        // row i of dF/dX_b and dF/dX_e
+            UNROLL for (int k = 0; k < B; k++) { rb[k] = 0; re[k] = 0; }
+            double flux = 0;
             UNROLL for (int j = 0; j < np; j++) {
                 Upw u;
                 if (!upwind(c, nb, bI, eI, j, u)) continue;
-                const int64_t U = u.upIsB ? bI : eI, D = u.upIsB ? eI : bI;
+                if (i == 0) continue;                   // the volume row carries no flux
+                const int64_t U = u.upIsB ? bI : eI;
                 const double xi = c.xi[j * nb + U], kr = c.kr[j * nb + U], mu = c.mu[j * nb + U];
                 const double tJ = T * kr / mu;
-                double xu[nc];
-                UNROLL for (int i = 0; i < nc; i++) xu[i] = c.xij[(size_t)(j * nc + i) * nb + U];
-                if (do_res && r > 0) flux += tJ * u.dP * xi * c.xij[(size_t)(j * nc + r - 1) * nb + U];
+                const double xij = c.xij[(size_t)(j * nc + i - 1) * nb + U];
+                flux += tJ * u.dP * xi * xij;
                 if (!do_jac) continue;
-                const double muP = c.muP[j * nb + U], xiP = c.xiP[j * nb + U];
+                const double tIJ = xij * xi * tJ;
                 const double wU = u.downExists ? 0.5 : 1.0, wD = u.downExists ? 0.5 : 0.0;
-                const double xt = xi * tJ, dP = u.dP;
-                // pressure-column coefficients (dFdXp col 0)
-                const double pU = tJ * xiP * dP - xt * muP / mu * dP - xt * wU * c.rhoP[j * nb + U] * dGamma;
-                const double pD = -xt * wD * c.rhoP[j * nb + D] * dGamma;
-                // q = coefficient vector dotted with the X column, per side
-                double qB = 0, qE = 0;
-                if (r == 0) {
-                    qB = xt + (u.upIsB ? pU : pD);
-                    qE = -xt + (u.upIsB ? pD : pU);
-                }
-                // the two register arrays are chosen element-wise with
-                // selects, never through a pointer, so both stay in VGPRs
-                const bool upB = u.upIsB;
-                const double kcoef = T / mu * xi * dP;
-                UNROLL for (int k = 0; k < np; k++) {
-                    const double xb = ownB ? Xo[k] : Xm[k], xe = ownB ? Xm[k] : Xo[k];
-                    const double cb = xt * c.dPcdS[(j * np + k) * nb + bI] + (upB ? kcoef * c.dKrdS[(j * np + k) * nb + U] : 0.0);
-                    const double ce = -xt * c.dPcdS[(j * np + k) * nb + eI] + (upB ? 0.0 : kcoef * c.dKrdS[(j * np + k) * nb + U]);
-                    qB += cb * xb;
-                    qE += ce * xe;
-                }
-                // upstream is the own cell when (own is b) == (up is b)
-                const bool upOwn = (ownB == upB);
-                double qU = 0, qD = 0;
-                UNROLL for (int mm = 0; mm < nc; mm++) {
-                    const int k = np + j * nc + mm;
-                    const double xU = upOwn ? Xo[k] : Xm[k], xD = upOwn ? Xm[k] : Xo[k];
-                    const double eu = -xt * wU * c.rhox[(size_t)(j * nc + mm) * nb + U] * dGamma +
-                                      tJ * c.xix[(size_t)(j * nc + mm) * nb + U] * dP -
-                                      xt * c.mux[(size_t)(j * nc + mm) * nb + U] / mu * dP;
-                    const double ed = -xt * wD * c.rhox[(size_t)(j * nc + mm) * nb + D] * dGamma;
-                    qU += eu * xU;
-                    qD += ed * xD;
-                }
-                if (upB) { qB += qU; qE += qD; } else { qE += qU; qB += qD; }
-                const double sI = tJ * xi * dP;   // k == i term, upstream side
-                UNROLL for (int i = 0; i < nc; i++) {
-                    const int k = np + j * nc + i;
-                    const double xU = upOwn ? Xo[k] : Xm[k];
-                    colB[i + 1] += xu[i] * qB + (upB ? sI * xU : 0.0);
-                    colE[i + 1] += xu[i] * qE + (upB ? 0.0 : sI * xU);
+                const double wb = u.upIsB ? wU : wD, we = u.upIsB ? wD : wU;
+                const double* Ab = c.aux + ((size_t)bI * np + j) * AUXJ;
+                const double* Ae = c.aux + ((size_t)eI * np + j) * AUXJ;
+                const double* AU = u.upIsB ? Ab : Ae;
+                const double* XU = c.dXsdXp + (size_t)U * REC + (np + j * nc + i - 1) * B;
+                const double lam = AU[3 * B], Tdp = T * u.dP;
+                UNROLL for (int r = 0; r < B; r++) {
+                    const double up = Tdp * (xij * AU[r] + lam * XU[r]);
+                    rb[r] += tIJ * (Ab[B + r] - dGamma * (wb * Ab[2 * B + r])) + (u.upIsB ? up : 0.0);
+                    re[r] += tIJ * (-Ae[B + r] - dGamma * (we * Ae[2 * B + r])) + (u.upIsB ? 0.0 : up);
                 }
             }
-            if (do_res && r > 0) res += ownB ? dt * flux : -dt * flux;
+            if (do_res && i > 0) res += ownB ? dt * flux : -dt * flux;
             if (do_jac) {
-                double* off = bsr.val + (size_t)s * B * B + r;
+                double* off = bsr.val + ((size_t)s * B + i) * B;
                 if (ownB) {                       // AddDiag(b, B); NewOffDiag(b, e, E)
-                    UNROLL for (int i = 0; i < B; i++) { diag[i] += dt * colB[i]; off[i * B] = dt * colE[i]; }
+                    UNROLL for (int r = 0; r < B; r++) { diag[r] += dt * rb[r]; off[r] = dt * re[r]; }
                 } else {                          // NewOffDiag(e, b, -B); AddDiag(e, -E)
-                    UNROLL for (int i = 0; i < B; i++) { off[i * B] = -(dt * colB[i]); diag[i] += -(dt * colE[i]); }
+                    UNROLL for (int r = 0; r < B; r++) { off[r] = -(dt * rb[r]); diag[r] += -(dt * re[r]); }
                 }
             }
         }
         if (do_jac) {
-            double* dg = bsr.val + (size_t)s0 * B * B + r;
-            UNROLL for (int i = 0; i < B; i++) dg[i * B] = diag[i];
+            double* dg = bsr.val + ((size_t)s0 * B + i) * B;
+            UNROLL for (int r = 0; r < B; r++) dg[r] = diag[r];
         }
         if (do_res) {
             const double out = -res;             // Dscalar(-1) :1170
-            resAbs[(size_t)n * B + r] = out;
-            if (bsr.rhs) bsr.rhs[(size_t)n * B + r] = out;   // CopyRhs
+            resAbs[(size_t)n * B + i] = out;
+            if (bsr.rhs) bsr.rhs[(size_t)n * B + i] = out;   // CopyRhs
             absV = fabs(res / c.rockVp[n]);
-            if (r > 0) absN = fabs(res / c.Nt[n]);
+            if (i > 0) absN = fabs(res / c.Nt[n]);
         }
     }
     if (!do_res) return;
@@ -300,7 +320,7 @@ __global__ void __launch_bounds__(256) get_solution(double* P, double* Ni, doubl
     double d[row];
     UNROLL for (int i = 0; i < row; i++) {
         double s = 0;
-        UNROLL for (int j = 0; j < col; j++) s += 1.0 * X[(size_t)(i * col + j) * nb + n] * uu[j];
+        UNROLL for (int j = 0; j < col; j++) s += 1.0 * X[((size_t)n * row + i) * col + j] * uu[j];
         d[i] = s;
     }
     double chop = 1;
@@ -379,8 +399,15 @@ hipError_t run_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st)
 hipError_t run_krpc_rock(const DevSat* s, double Pref, double cp1, double cp2, const CellIO& c, hipStream_t st)
 {
     if (c.nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(krpc_rock, dim3((unsigned)((c.nb + 255) / 256)), dim3(256), 0, st, s, Pref, cp1, cp2, c);
-    return hipGetLastError();
+    const int nc = c.nc;
+    const int G = 64 / (nc + 1);
+    const unsigned nblk = (unsigned)((c.nb + G - 1) / G);
+#define X_(N) case N: hipLaunchKernelGGL(krpc_aux<N>, dim3(nblk), dim3(64), 0, st, s, Pref, cp1, cp2, c); return hipGetLastError();
+    switch (nc) {
+    X_(3) X_(4) X_(5) X_(6) X_(7) X_(8) X_(9) X_(10) X_(11) X_(12) X_(13)
+    default: return hipErrorInvalidValue;
+    }
+#undef X_
 }
 
 template <int NCB>

@@ -26,9 +26,10 @@ using namespace ocpx;
 namespace {
 
 struct Field {
-    void* d = nullptr;   // device SoA [len][nb]
+    void* d = nullptr;   // device [len][nb], or [nb][len] when aos
     int len = 0;         // entries per cell
     int esz = 8;         // element size
+    bool aos = false;    // cell-major on the device too (dXsdXp)
 };
 
 struct TimingAcc {
@@ -55,6 +56,7 @@ struct ocpx_ctx {
     bool ftype_override = false;
     // geometry
     double *v = nullptr, *depth = nullptr, *poroInit = nullptr, *trans = nullptr;
+    double* aux = nullptr;       // connection aux record [nb][3][3b+1]
     int32_t *cb = nullptr, *ce = nullptr;
     // PE results
     int32_t* pe_NP = nullptr;
@@ -69,7 +71,11 @@ struct ocpx_ctx {
     double* norms4 = nullptr;
     double *lam = nullptr, *out = nullptr, *gT = nullptr, *gV = nullptr;
     bool flashed = false, assembled = false;
+    bool aux_valid = false;      // krpc_aux ran on the current flash state
     int grid_waves = 0;
+    int pe_lanes = 1;            // batched-flash mapping: lanes per cell (1, 8, 16)
+    int pe_lanes_auto = 0;       // the autotuner's choice, 0 if it has not run
+    int team_blocks[2] = {0, 0}; // persistent grid of the team kernels, T = 8 / 16
     // timing
     bool timing = false;
     TimingAcc tacc[5];
@@ -176,6 +182,7 @@ CellIO cell_io(ocpx_ctx* c)
     io.dKrdS = D(c, OCPX_F_DKRDS); io.dPcdS = D(c, OCPX_F_DPCDS);
     io.poro = D(c, OCPX_F_PORO); io.poroP = D(c, OCPX_F_POROP); io.rockVp = D(c, OCPX_F_ROCKVP);
     io.v = c->v; io.depth = c->depth; io.poroInit = c->poroInit;
+    io.aux = c->aux;
     return io;
 }
 
@@ -230,9 +237,16 @@ int launch_pe(ocpx_ctx* c, int mode)
     io.queue = c->queue;
     if (mode == OCPX_FLASH_BATCHED) HIPCHK(c, run_fill_u32(c->queue, 0u, 1, c->st));
     TimerScope t(c, 0);
-    HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
+    if (mode == OCPX_FLASH_BATCHED && c->pe_lanes > 1) {
+        const int T = c->pe_lanes;
+        HIPCHK(c, run_pe_team(c->NC, T, c->df, io, true, c->team_blocks[T == 8 ? 0 : 1], c->st));
+    } else {
+        HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
+    }
     return OCPX_OK;
 }
+
+int default_lanes(int NC) { return NC <= 4 ? 1 : (NC <= 8 ? 8 : 16); }
 
 int launch_tail(ocpx_ctx* c)
 {
@@ -259,6 +273,7 @@ int launch_krpc_rock(ocpx_ctx* c)
 {
     TimerScope t(c, 2);
     HIPCHK(c, run_krpc_rock(c->ds, c->rock.Pref, c->rock.cp1, c->rock.cp2, cell_io(c), c->st));
+    c->aux_valid = c->flashed;
     return OCPX_OK;
 }
 
@@ -367,6 +382,7 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
     for (int fi = 0; fi < OCPX_F_NFIELDS; fi++) {
         c->f[fi].len = (int)flen(fi, c->NC);
         c->f[fi].esz = fesz(fi);
+        c->f[fi].aos = fi == OCPX_F_DSEC_DPRI;
         if (c->f[fi].esz == 8) { double* p; ALLOC(p, (size_t)c->f[fi].len * c->nb); c->f[fi].d = p; }
         else { int32_t* p; ALLOC(p, (size_t)c->f[fi].len * c->nb); c->f[fi].d = p; }
     }
@@ -374,6 +390,7 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
     ALLOC(c->cb, c->nconn); ALLOC(c->ce, c->nconn); ALLOC(c->trans, c->nconn);
     ALLOC(c->pe_NP, c->nb); ALLOC(c->pe_nu, c->nb); ALLOC(c->pe_x, (size_t)2 * c->NC * c->nb);
     ALLOC(c->queue, 4);
+    ALLOC(c->aux, (size_t)c->nb * 3 * (3 * (c->nc + 1) + 1));
     // BSR in OCPMatrix push order: diagonal first, then per connection in
     // index order, row e gets (e, b) and row b gets (b, e)
     c->nnzb = c->nb + 2 * c->nconn;
@@ -426,6 +443,13 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const int per_cu = pe_batched_blocks_per_cu(c->NC);
     c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
+    for (int q = 0; q < 2; q++) {
+        const int T = q == 0 ? 8 : 16;
+        if (c->NC > T) continue;
+        const int b = pe_team_blocks_per_cu(c->NC, T);
+        c->team_blocks[q] = (ncu > 0 ? ncu : 256) * (b > 0 ? b : 4);
+    }
+    c->pe_lanes = default_lanes(c->NC);
     *out = c;
     return OCPX_OK;
 }
@@ -456,12 +480,17 @@ int ocpx_set_field(ocpx_ctx* c, int field, const void* host)
     if (!c || !host || field < 0 || field >= OCPX_F_NFIELDS) return fail(c, OCPX_E_ARG, "bad field");
     const Field& F = c->f[field];
     const size_t n = c->nb, L = F.len;
-    std::vector<char> tmp(n * L * F.esz);
-    if (F.esz == 8) to_soa((const uint64_t*)host, (uint64_t*)tmp.data(), n, L);
-    else to_soa((const uint32_t*)host, (uint32_t*)tmp.data(), n, L);
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(F.d, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    if (F.aos) {
+        HIPCHK(c, hipMemcpy(F.d, host, n * L * F.esz, hipMemcpyHostToDevice));
+    } else {
+        std::vector<char> tmp(n * L * F.esz);
+        if (F.esz == 8) to_soa((const uint64_t*)host, (uint64_t*)tmp.data(), n, L);
+        else to_soa((const uint32_t*)host, (uint32_t*)tmp.data(), n, L);
+        HIPCHK(c, hipMemcpy(F.d, tmp.data(), tmp.size(), hipMemcpyHostToDevice));
+    }
     if (field == OCPX_F_DSEC_DPRI) c->flashed = true;   // dXsdXp supplied by the caller
+    c->aux_valid = false;
     if (field == OCPX_F_FTYPE_IN) {
         const int32_t* fi = (const int32_t*)host;
         bool any = false;
@@ -476,8 +505,12 @@ int ocpx_get_field(ocpx_ctx* c, int field, void* host)
     if (!c || !host || field < 0 || field >= OCPX_F_NFIELDS) return fail(c, OCPX_E_ARG, "bad field");
     const Field& F = c->f[field];
     const size_t n = c->nb, L = F.len;
-    std::vector<char> tmp(n * L * F.esz);
     HIPCHK(c, hipStreamSynchronize(c->st));
+    if (F.aos) {
+        HIPCHK(c, hipMemcpy(host, F.d, n * L * F.esz, hipMemcpyDeviceToHost));
+        return OCPX_OK;
+    }
+    std::vector<char> tmp(n * L * F.esz);
     HIPCHK(c, hipMemcpy(tmp.data(), F.d, tmp.size(), hipMemcpyDeviceToHost));
     if (F.esz == 8) to_aos((const uint64_t*)tmp.data(), (uint64_t*)host, n, L);
     else to_aos((const uint32_t*)tmp.data(), (uint32_t*)host, n, L);
@@ -495,6 +528,63 @@ int ocpx_phase_equilibrium(ocpx_ctx* c, int mode)
 {
     if (!c || (mode != OCPX_FLASH_PLAIN && mode != OCPX_FLASH_BATCHED)) return fail(c, OCPX_E_ARG, "bad mode");
     return launch_pe(c, mode);
+}
+
+int ocpx_set_flash_mapping(ocpx_ctx* c, int lanes)
+{
+    if (!c) return OCPX_E_ARG;
+    if (lanes == 0) lanes = c->pe_lanes_auto ? c->pe_lanes_auto : default_lanes(c->NC);
+    if (lanes != 1 && lanes != 8 && lanes != 16) return fail(c, OCPX_E_ARG, "lanes per cell must be 1, 8 or 16");
+    if (lanes > 1 && c->NC > lanes) return fail(c, OCPX_E_ARG, "a team of 8 lanes holds at most 8 components");
+    if (lanes != c->pe_lanes) destroy_graph(c);
+    c->pe_lanes = lanes;
+    return OCPX_OK;
+}
+
+int ocpx_get_flash_mapping(ocpx_ctx* c, int32_t* lanes)
+{
+    if (!c || !lanes) return OCPX_E_ARG;
+    *lanes = c->pe_lanes;
+    return OCPX_OK;
+}
+
+int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
+{
+    if (!c) return OCPX_E_ARG;
+    static const int cand[3] = {1, 8, 16};
+    double t[3] = {0, 0, 0};
+    const int keep = c->pe_lanes;
+    const bool tm = c->timing;
+    c->timing = false;
+    int best = keep;
+    double best_ms = 0;
+    hipEvent_t a = nullptr, b = nullptr;
+    HIPCHK(c, hipEventCreate(&a));
+    HIPCHK(c, hipEventCreate(&b));
+    int rc = OCPX_OK;
+    for (int q = 0; q < 3 && !rc; q++) {
+        if (cand[q] > 1 && c->NC > cand[q]) continue;
+        c->pe_lanes = cand[q];
+        for (int rep = 0; rep < 2 && !rc; rep++) {      // first run warms caches and code
+            hipEventRecord(a, c->st);
+            rc = launch_pe(c, OCPX_FLASH_BATCHED);
+            hipEventRecord(b, c->st);
+        }
+        if (rc) break;
+        if (hipEventSynchronize(b) != hipSuccess) { rc = fail(c, OCPX_E_HIP, "autotune sync"); break; }
+        float e = 0;
+        hipEventElapsedTime(&e, a, b);
+        t[q] = e;
+        if (best_ms == 0 || e < best_ms) { best_ms = e; best = cand[q]; }
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    c->timing = tm;
+    c->pe_lanes = keep;
+    if (rc) return rc;
+    if (ms) for (int q = 0; q < 3; q++) ms[q] = t[q];
+    c->pe_lanes_auto = best;
+    return ocpx_set_flash_mapping(c, best);
 }
 
 int ocpx_get_pe_result(ocpx_ctx* c, int32_t* NP, double* nu, double* x)
@@ -525,6 +615,7 @@ int ocpx_flash_der(ocpx_ctx* c, int mode)
     rc = launch_tail(c);
     if (rc) return rc;
     c->flashed = true;
+    c->aux_valid = false;
     return OCPX_OK;
 }
 
@@ -560,6 +651,7 @@ int ocpx_assemble(ocpx_ctx* c, double dt)
 {
     if (!c) return OCPX_E_ARG;
     if (!c->flashed) return fail(c, OCPX_E_STATE, "ocpx_assemble before ocpx_flash_der");
+    if (!c->aux_valid) return fail(c, OCPX_E_STATE, "ocpx_assemble needs ocpx_krpc_der after the flash");
     int rc = launch_res_asm(c, dt, true, true);
     if (rc) return rc;
     c->assembled = true;
@@ -598,6 +690,7 @@ int ocpx_get_solution(ocpx_ctx* c, const double* u, double dSmax)
     HIPCHK(c, hipMemcpy(c->lam, u, c->nb * B * 8, hipMemcpyHostToDevice));
     HIPCHK(c, run_get_solution(c->nc, D(c, OCPX_F_P), D(c, OCPX_F_NI), D(c, OCPX_F_S), D(c, OCPX_F_XIJ),
                                D(c, OCPX_F_DSEC_DPRI), c->lam, dSmax, c->nb, c->st));
+    c->aux_valid = false;
     return OCPX_OK;
 }
 
@@ -642,6 +735,7 @@ int ocpx_reset_to_last_step(ocpx_ctx* c)
 {
     if (!c) return OCPX_E_ARG;
     if (!c->have_last) return fail(c, OCPX_E_STATE, "ocpx_reset_to_last_step before ocpx_update_last_step");
+    c->aux_valid = false;
     for (int fid : kLastStepFields)
         HIPCHK(c, hipMemcpyAsync(c->f[fid].d, c->lf[fid].d, field_bytes(c, fid), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->f[OCPX_F_NI].d, c->f[OCPX_F_LNI].d, field_bytes(c, OCPX_F_NI),

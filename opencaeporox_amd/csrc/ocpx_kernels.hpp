@@ -49,11 +49,15 @@ struct TailIO {
     // outputs (BulkVarSet mirror)
     uint32_t* phaseExist;
     double *S, *xij, *rho, *xi, *mu, *rhoP, *xiP, *muP, *rhox, *xix, *mux;
-    double *Nt, *vf, *vfP, *vfi, *dXsdXp;
+    double *Nt, *vf, *vfP, *vfi, *dXsdXp;   // dXsdXp cell-major [nb][np*b][b]
     uint32_t* skip_flag;
     double *skip_minEig, *skip_P, *skip_T, *skip_zi;
 };
 
+// team mapping (pe_team.hip): T lanes per cell, T = 8 or 16
+hipError_t run_pe_team(int NC, int T, const DevFluid* f, const PeIO& io, bool batched, int grid_blocks,
+                       hipStream_t st);
+int pe_team_blocks_per_cu(int NC, int T);
 hipError_t run_pe(int NC, const DevFluid* f, const PeIO& io, int mode, int grid_waves,
                   hipStream_t st);
 hipError_t run_tail(int NC, const DevFluid* f, const TailIO& io, hipStream_t st);
@@ -67,6 +71,7 @@ struct CellIO {
     const uint32_t* phaseExist;
     double *kr, *Pc, *Pj, *dKrdS, *dPcdS, *poro, *poroP, *rockVp;
     const double *v, *depth, *poroInit;
+    double* aux;                // [nb][3][3b+1] connection aux record (krpc_aux)
 };
 
 // block-CSR and cell adjacency

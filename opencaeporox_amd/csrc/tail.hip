@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(64) flash_tail(const DevFluid* __restrict__ fp
     const double iVf = 1 / Vf;
 
     // dXsdXp rows: S_j (j < 3), then x_ij at row np + j*nc + i; all written
-    auto W = [&](int row, int col, double v) { io.dXsdXp[at(row * ncol + col)] = v; };
+    auto W = [&](int row, int col, double v) { io.dXsdXp[((size_t)cell * 3 * ncol + row) * ncol + col] = v; };
     double vfP;
     if (NP == 1) {
         // the existing phase j is a runtime value: pick its data with selects

@@ -104,6 +104,21 @@ class IsoTFIM:
                   "pe result")
         return NP, nu.reshape(-1, 2), x.reshape(-1, 2, self.Nc)
 
+    def set_flash_mapping(self, lanes_per_cell):
+        self._chk(self.lib.ocpx_set_flash_mapping(self.h, lanes_per_cell), "flash mapping")
+
+    def flash_mapping(self):
+        v = C.c_int32()
+        self._chk(self.lib.ocpx_get_flash_mapping(self.h, C.byref(v)), "flash mapping")
+        return v.value
+
+    def autotune_flash(self):
+        """Times the batched phase-equilibrium kernel per mapping on the
+        current state; keeps the fastest.  Returns {lanes_per_cell: ms}."""
+        ms = np.zeros(3)
+        self._chk(self.lib.ocpx_autotune_flash(self.h, ms.ctypes.data), "autotune")
+        return {k: v for k, v in zip((1, 8, 16), ms) if v > 0}
+
     def CalFlash(self, mode=PLAIN):
         self._chk(self.lib.ocpx_flash_der(self.h, mode), "flash")
 

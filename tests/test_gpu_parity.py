@@ -65,15 +65,16 @@ def _compare_flash(dev, st, Nc, tag, others, max_mismatch=0.02):
     return ~mism
 
 
-@pytest.mark.parametrize("mode", [PLAIN, BATCHED])
-def test_flash_config1(oracle, oracle_fma, mode):
+@pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 8), (BATCHED, 16)])
+def test_flash_config1(oracle, oracle_fma, mode, lanes):
     wl = W.config1()
     dev = IsoTFIM.from_workload(wl)
+    dev.set_flash_mapping(lanes)
     st = state_from_workload(wl)
     others = perturbed_oracle_states(oracle, oracle_fma, wl.fluid, st)
     dev.CalFlash(mode)
     oracle.cal_flash(wl.fluid, st)
-    same = _compare_flash(dev, st, wl.Nc, f"C1 mode {mode}", others)
+    same = _compare_flash(dev, st, wl.Nc, f"C1 mode {mode} lanes {lanes}", others)
     # SkipPSA next-step data
     keep = same & ~rounding_sensitive(st.ill)
     assert np.array_equal(dev.get("SKIP_FLAG")[keep], st.a["SKIP_FLAG"][keep])
@@ -81,11 +82,21 @@ def test_flash_config1(oracle, oracle_fma, mode):
     assert ok, f"minEig worst {worst}"
 
 
-@pytest.mark.parametrize("Nc", [3, 4, 5, 9, 12])
-def test_phase_equilibrium_cells(oracle, oracle_fma, Nc):
-    wl, z = W.flash_cells(3000, Nc, seed=10 + Nc)
+PE_CASES = [(Nc, m, lanes) for Nc in (3, 4, 5, 9, 12)
+            for m, lanes in ((PLAIN, 1), (BATCHED, 1), (BATCHED, 8 if Nc <= 8 else 16))]
+
+
+@pytest.mark.parametrize("Nc,mode,lanes", PE_CASES)
+def test_phase_equilibrium_cells(oracle, oracle_fma, Nc, mode, lanes):
+    """Phase count, split and flash type per cell against the oracle, for
+    the one-cell-per-lane kernels and the team mapping (one component per
+    lane)."""
+    # pressures across both phase regions; seed 44 gives the 3-component
+    # fluid a two-phase region there
+    wl, z = W.flash_cells(3000, Nc, seed=44 if Nc == 3 else 10 + Nc, P=np.linspace(5.0, 250.0, 3000))
     dev = IsoTFIM.from_workload(wl)
-    dev.PhaseEquilibrium(PLAIN)
+    dev.set_flash_mapping(lanes)
+    dev.PhaseEquilibrium(mode)
     NP, nu, x = dev.pe_result()
     P = wl.state["P"]
     T = wl.state["T"] + W.CONV4
@@ -98,6 +109,7 @@ def test_phase_equilibrium_cells(oracle, oracle_fma, Nc):
     assert bad.size == 0, f"Nc={Nc}: NP differs in {bad.size} cells {bad[:10]}"
     assert mism.mean() < 0.02
     two = (NP == 2) & (NPo == 2) & ~sens
+    assert two.sum() > 100, "workload must contain well-conditioned two-phase cells"
     runs = [oracle_fma.phase_equilibrium(wl.fluid, P, T, z),
             oracle.phase_equilibrium(wl.fluid, P * (1 + 2.0 ** -50), T, z)]
     for name, dv, ov, k in (("nu", nu, nuo, 1), ("x", x, xo, 2)):
@@ -126,9 +138,11 @@ def test_phase_equilibrium_cells(oracle, oracle_fma, Nc):
 
 
 def test_batched_equals_plain():
+    """The work queue changes the order cells are processed in, nothing else."""
     wl, z = W.flash_cells(5000, 9, seed=77)
     a = IsoTFIM.from_workload(wl)
     b = IsoTFIM.from_workload(wl)
+    b.set_flash_mapping(1)
     a.CalFlash(PLAIN)
     b.CalFlash(BATCHED)
     for f in FLASH_OUT:
@@ -199,14 +213,15 @@ def _clean_grid(seed=31, Nc=3, nx=4, ny=3, nz=2):
     return W.Workload("clean grid", fl, W.make_sat(), rock, grid, stt)
 
 
-@pytest.mark.parametrize("mode", [PLAIN, BATCHED])
-def test_newton_passes_end_to_end(oracle, oracle_fma, mode):
+@pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 8)])
+def test_newton_passes_end_to_end(oracle, oracle_fma, mode, lanes):
     """Device flash + assembly against the oracle's on a grid whose cells are
     all well-conditioned, then GetSolution and a second pass from the
     two-phase history."""
     wl = _clean_grid()
     dt = 1.0
     dev = IsoTFIM.from_workload(wl)
+    dev.set_flash_mapping(lanes)
     st = state_from_workload(wl)
     others = perturbed_oracle_states(oracle, oracle_fma, wl.fluid, st)
     dev.CalFlash(mode)
@@ -260,6 +275,16 @@ def test_jt_lambda_dot_product():
             Jv[r * b:(r + 1) * b] += val[s] @ v[col[s] * b:(col[s] + 1) * b]
     JTl = dev.jt_lambda(lam)
     assert abs(Jv @ lam - v @ JTl) <= 1e-10 * (abs(Jv @ lam) + np.abs(Jv).dot(np.abs(lam)))
+
+
+def test_autotuner_picks_a_valid_mapping():
+    wl, _ = W.flash_cells(20000, 6, seed=3)
+    dev = IsoTFIM.from_workload(wl)
+    ms = dev.autotune_flash()
+    assert set(ms) == {1, 8, 16} and all(v > 0 for v in ms.values())
+    assert dev.flash_mapping() == min(ms, key=ms.get)
+    dev.set_flash_mapping(0)
+    assert dev.flash_mapping() == min(ms, key=ms.get)
 
 
 def test_graph_replay_matches_eager():
