@@ -45,6 +45,9 @@ def parse():
     p.add_argument("--assembly", type=str, default="100x100x50",
                    help="grid of the assembly measurement, '' to skip")
     p.add_argument("--modes", type=str, default="plain,batched")
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="only copy a buffer of this many MiB on the GPU (calibrates the "
+                        "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     return p.parse_args()
 
 
@@ -167,6 +170,15 @@ def main():
 
     def sync():
         torch.cuda.synchronize()
+
+    if a.copy_mib:
+        x = torch.ones(a.copy_mib << 17, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        sync()
+        log(f"copied {a.copy_mib} MiB three times")
+        return
 
     dt = 1.0
     t = time.perf_counter()

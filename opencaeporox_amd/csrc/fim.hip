@@ -147,6 +147,16 @@ __global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, do
     }
 }
 
+// Blocks are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8).
+// Renumber them so each XCD works through one contiguous chunk of rows and
+// the neighbour records a row reads stay in that XCD's L2.
+__device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t nblk)
+{
+    constexpr int64_t X = 8;
+    const int64_t x = b % X, k = b / X, q = nblk / X, r = nblk % X;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // --------------------------------------------------------- residual + Jacobian
 // Upwinding of one phase (OCPConvection01::CalFlux :37-78).  Returns false
 // when the phase carries no flux.
@@ -179,8 +189,10 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
     const int lane = threadIdx.x;
     const int g = lane / B, i = lane % B;              // i: equation (block row)
     const size_t nb = c.nb;
-    const int64_t n = (int64_t)blockIdx.x * G + g;
-    const bool valid = (g < G) && (n < c.nb);
+    const int64_t blk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int64_t p = blk * G + g;                     // position in the row order
+    const bool valid = (g < G) && (p < c.nb);
+    const int64_t n = valid ? (bsr.order ? (int64_t)bsr.order[p] : p) : 0;
     __shared__ double sV[64], sN[64];
     __shared__ int64_t iV[64], iN[64];
 

@@ -6,6 +6,7 @@
 // device-resident and component-major.  Host arrays cross the boundary in
 // the reference's BulkVarSet layout (cell-major) and are transposed here.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,6 +66,7 @@ struct ocpx_ctx {
     // linear system
     int64_t* rowptr = nullptr;
     int32_t *colind = nullptr, *tslot = nullptr, *nbr_conn = nullptr;
+    int32_t* order = nullptr;    // assembly row order (null = natural)
     double *val = nullptr, *rhs = nullptr, *resAbs = nullptr;
     double* bmax = nullptr;
     int64_t* barg = nullptr;
@@ -191,6 +193,7 @@ BsrIO bsr_io(ocpx_ctx* c)
     BsrIO b{};
     b.rowptr = c->rowptr; b.colind = c->colind; b.tslot = c->tslot; b.nbr_conn = c->nbr_conn;
     b.val = c->val; b.rhs = c->rhs;
+    b.order = c->order;
     return b;
 }
 
@@ -247,6 +250,45 @@ int launch_pe(ocpx_ctx* c, int mode)
 }
 
 int default_lanes(int NC) { return NC <= 4 ? 1 : (NC <= 8 ? 8 : 16); }
+
+// Row order for the assembly.  When the connections are those of a natural
+// Cartesian numbering (every e - b is 1, nx or nx*ny, x and y connections do
+// not wrap), rows go in pencils: 16 x 16 tiles of the xy plane, each swept
+// through z, so the z-neighbours a row reads were touched a plane earlier
+// and are still in the XCD's L2.  Otherwise the natural order is kept.
+std::vector<int32_t> pencil_order(int64_t nb, int64_t nconn, const int32_t* cb, const int32_t* ce)
+{
+    int64_t d[3] = {0, 0, 0};
+    int nd = 0;
+    for (int64_t k = 0; k < nconn; k++) {
+        const int64_t v = (int64_t)ce[k] - cb[k];
+        bool seen = false;
+        for (int q = 0; q < nd; q++) seen = seen || d[q] == v;
+        if (seen) continue;
+        if (nd == 3 || v <= 0) return {};
+        d[nd++] = v;
+    }
+    if (nd != 3) return {};
+    std::sort(d, d + 3);
+    const int64_t nx = d[1], nxy = d[2];
+    if (d[0] != 1 || nxy % nx != 0 || nb % nxy != 0) return {};
+    const int64_t ny = nxy / nx, nz = nb / nxy;
+    if (nz < 2 || ny < 2 || nx < 2) return {};
+    for (int64_t k = 0; k < nconn; k++) {
+        const int64_t b = cb[k], e = ce[k];
+        if (e - b == 1 && b % nx == nx - 1) return {};
+        if (e - b == nx && (b / nx) % ny == ny - 1) return {};
+    }
+    const int64_t TX = 16, TY = 16;
+    std::vector<int32_t> ord;
+    ord.reserve(nb);
+    for (int64_t y0 = 0; y0 < ny; y0 += TY)
+        for (int64_t x0 = 0; x0 < nx; x0 += TX)
+            for (int64_t z = 0; z < nz; z++)
+                for (int64_t y = y0; y < std::min(ny, y0 + TY); y++)
+                    for (int64_t x = x0; x < std::min(nx, x0 + TX); x++) ord.push_back((int32_t)(z * nxy + y * nx + x));
+    return ord;
+}
 
 int launch_tail(ocpx_ctx* c)
 {
@@ -417,6 +459,8 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
     ALLOC(c->bmax, 2 * nblk); ALLOC(c->barg, 2 * nblk); ALLOC(c->norms4, 4);
     ALLOC(c->lam, (size_t)c->nb * B); ALLOC(c->out, (size_t)c->nb * B);
     ALLOC(c->gT, c->nconn); ALLOC(c->gV, c->nb);
+    const std::vector<int32_t> ord = pencil_order(c->nb, c->nconn, g->conn_b, g->conn_e);
+    if (!ord.empty()) ALLOC(c->order, c->nb);
 #undef ALLOC
     bool okc = hipMemcpy(c->rowptr, rp.data(), rp.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
                hipMemcpy(c->colind, col.data(), col.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
@@ -425,6 +469,8 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
                hipMemcpy(c->v, g->v, c->nb * 8, hipMemcpyHostToDevice) == hipSuccess &&
                hipMemcpy(c->depth, g->depth, c->nb * 8, hipMemcpyHostToDevice) == hipSuccess &&
                hipMemcpy(c->poroInit, g->poroInit, c->nb * 8, hipMemcpyHostToDevice) == hipSuccess;
+    if (okc && c->order)
+        okc = hipMemcpy(c->order, ord.data(), ord.size() * 4, hipMemcpyHostToDevice) == hipSuccess;
     if (okc && c->nconn > 0)
         okc = hipMemcpy(c->cb, g->conn_b, c->nconn * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(c->ce, g->conn_e, c->nconn * 4, hipMemcpyHostToDevice) == hipSuccess &&

@@ -82,6 +82,7 @@ struct BsrIO {
     const int32_t* nbr_conn;    // [nnzb] connection of the slot (-1 = diagonal)
     double* val;                // [nnzb][b][b]
     double* rhs;                // [nb][b]
+    const int32_t* order;       // [nb] row processing order, or null (natural)
 };
 
 struct ConnIO {

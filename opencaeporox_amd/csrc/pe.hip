@@ -15,9 +15,12 @@
 //                  queue as soon as their cell is done, so lanes stuck on
 //                  near-saturation cells (hundreds of SSM steps) do not idle
 //                  the rest of the wavefront.
-// Algebraically equivalent forms replace some of the reference's
-// expressions (log-space SSM residuals, K = phi1/phi0 in the split SSM);
-// they change results only at rounding level.
+// The residuals and updates are the reference's literal expressions
+// (fugacities phi*x*P, their ratios and logs), so degenerate trajectories
+// (K far from 1, Rachford-Rice fractions outside [0, 1]) overflow and
+// stop the way the reference's do.  In pe_batched a lane that reaches a
+// Newton step waits for a batch of them (NR_BATCH) before the wavefront
+// runs the Newton branch.
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 
@@ -38,6 +41,7 @@ enum : int {
 
 constexpr int MAX_SPLIT_RESTARTS = 64;   // the reference's restart loop has no cap
 constexpr int MAX_STEPS = 200000;        // hard bound on a cell's steps (flagged if hit)
+constexpr int NR_BATCH = 16;             // lanes a Newton step waits for (pe_batched)
 constexpr double KTOL = 1E-4, DYTOL = 1E-6;   // include/OCPPhaseEquilibrium.hpp:43-44
 
 template <int NC>
@@ -48,8 +52,7 @@ struct PeLane {
     int cnt[8];
     double z[NC];
     EosCell<NC> ec;
-    EosMix<NC> mx;
-    double y[NC], ev[NC];
+    double y[NC];
     double a1[NC], a2[NC], a3[NC], a4[NC], a5[NC];
 };
 
@@ -330,6 +333,10 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
 {
     // every cell ends: a legitimate flash needs far fewer steps than this
     if (++L.steps > MAX_STEPS) { L.cnt[7] |= 0x20000000; L.st = ST_DONE; return; }
+    // mixture terms and ln phi of the composition evaluated in this step
+    // (step-local, so they are not carried in registers across steps)
+    EosMix<NC> mx;
+    double ev[NC];
     // PRE: state-specific preparation of y (and the NR solves)
     bool eval = true;
     switch (L.st) {
@@ -363,13 +370,14 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
         break;
     }
     case ST_T_NR: {   // Jacobian (:523-541) solved through its lower triangle (dsysv_ 'U', :493)
+        eos_mix<NC>(f, L.ec, L.y, mx);              // mixture at the current Y (as evaluated last step)
         double Zx[NC];
-        const double den = z_den(L.mx), bc = z_bcoef(L.mx);
-        UNROLL for (int k = 0; k < NC; k++) Zx[k] = ((L.mx.B - L.mx.Z) * (2 * L.mx.s[k]) + bc * L.ec.Bi[k]) / den;
+        const double den = z_den(mx), bc = z_bcoef(mx);
+        UNROLL for (int k = 0; k < NC; k++) Zx[k] = ((mx.B - mx.Z) * (2 * mx.s[k]) + bc * L.ec.Bi[k]) / den;
         double S[NC][NC], rhs[1][NC];
         UNROLL for (int i = 0; i < NC; i++) {
             double row[NC];
-            lnfugx_row<NC>(f, L.ec, L.mx, Zx, L.y, L.P, i, row);
+            lnfugx_row<NC>(f, L.ec, mx, Zx, L.y, L.P, i, row);
             double t = 0;
             UNROLL for (int k = 0; k < NC; k++) t += L.y[k] * row[k];
             UNROLL for (int j = 0; j <= i; j++) {
@@ -444,15 +452,15 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
 
     // COMMON EVALUATOR: ln phi at y
     if (eval) {
-        eos_mix<NC>(f, L.ec, L.y, L.mx);
-        eos_lnphi<NC>(L.ec, L.mx, L.ev);
+        eos_mix<NC>(f, L.ec, L.y, mx);
+        eos_lnphi<NC>(L.ec, mx, ev);
         L.cnt[5]++;
     }
 
     // POST
     switch (L.st) {
     case ST_FEED:   // di = phi[Id] * x (:403-408), fug[Id] = di * P
-        UNROLL for (int i = 0; i < NC; i++) L.a1[i] = exp(L.ev[i]) * L.z[i];
+        UNROLL for (int i = 0; i < NC; i++) L.a1[i] = exp(ev[i]) * L.z[i];
         L.k = 0;
         L.st = L.lNP == 0 ? ST_T_START : ST_A_START;   // PhaseStable :279-291
         break;
@@ -460,7 +468,7 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
     case ST_T_SSM: {   // Se = sum ln(fugSta / fugId * Yt)^2 (:429-432, :448-451)
         double Se = 0;
         UNROLL for (int i = 0; i < NC; i++) {
-            const double phS = exp(L.ev[i]);
+            const double phS = exp(ev[i]);
             const double r = log((phS * L.y[i] * L.P) / (L.a1[i] * L.P) * L.Yt);
             Se += r * r;
             L.a3[i] = -r;                          // resSTA for StableNR
@@ -480,7 +488,7 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
     case ST_T_NR: {
         double e2 = 0;
         UNROLL for (int i = 0; i < NC; i++) {
-            L.a3[i] = log((L.a1[i] * L.P) / ((exp(L.ev[i]) * L.y[i] * L.P) * L.Yt));
+            L.a3[i] = log((L.a1[i] * L.P) / ((exp(ev[i]) * L.y[i] * L.P) * L.Yt));
             e2 += L.a3[i] * L.a3[i];
         }
         L.cnt[1]++;
@@ -496,13 +504,13 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
         // di = fugId / (fugSta Yt), ks *= di, Se, Sk (:347-356)
         double Se = 0, Sk = 0;
         UNROLL for (int i = 0; i < NC; i++) {
-            const double di = (L.a1[i] * L.P) / ((exp(L.ev[i]) * L.y[i] * L.P) * L.Yt);
+            const double di = (L.a1[i] * L.P) / ((exp(ev[i]) * L.y[i] * L.P) * L.Yt);
             L.a4[i] *= di;
             Se += (di - 1) * (di - 1);
             const double lk = log(L.a4[i]);
             Sk += lk * lk;
             L.a3[i] = L.y[i];
-            L.a2[i] = L.ev[i];
+            L.a2[i] = ev[i];
         }
         L.Sk = Sk;
         L.cnt[0]++;
@@ -513,7 +521,7 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
         break;
     }
     case ST_S_SSM_A:
-        UNROLL for (int i = 0; i < NC; i++) L.a5[i] = L.ev[i];
+        UNROLL for (int i = 0; i < NC; i++) L.a5[i] = ev[i];
         L.st = ST_S_SSM_B;
         break;
     case ST_S_SSM_B: {
@@ -525,7 +533,7 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
         // ratio, K <- phi1/phi0 (identical overflow behaviour included)
         double Knew[NC];
         UNROLL for (int i = 0; i < NC; i++) {
-            const double phi0 = exp(L.a5[i]), phi1 = exp(L.ev[i]);
+            const double phi0 = exp(L.a5[i]), phi1 = exp(ev[i]);
             const double q = (phi1 * L.a3[i] * L.P) / (phi0 * L.a2[i] * L.P) - 1;
             Se += q * q;
             Knew[i] = phi1 / phi0;
@@ -535,19 +543,19 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
         UNROLL for (int i = 0; i < NC; i++) L.a4[i] = Knew[i];
         // SplitNR starts from the x0, x1 just evaluated: ln(f1/f0) from
         // ln phi1 = ev and ln phi0 = a5
-        if (last) split_nr_begin(f, L, io, L.ev);
+        if (last) split_nr_begin(f, L, io, ev);
         else L.st = ST_S_SSM_A;
         break;
     }
     case ST_S_NR: {
         double lnphi0[NC];
-        UNROLL for (int i = 0; i < NC; i++) { lnphi0[i] = L.ev[i]; L.y[i] = L.a3[i]; }
-        eos_mix<NC>(f, L.ec, L.y, L.mx);
-        eos_lnphi<NC>(L.ec, L.mx, L.ev);
+        UNROLL for (int i = 0; i < NC; i++) { lnphi0[i] = ev[i]; L.y[i] = L.a3[i]; }
+        eos_mix<NC>(f, L.ec, L.y, mx);
+        eos_lnphi<NC>(L.ec, mx, ev);
         L.cnt[5]++;
         double e2 = 0;
         UNROLL for (int i = 0; i < NC; i++) {
-            L.a5[i] = log((exp(L.ev[i]) * L.a3[i] * L.P) / (exp(lnphi0[i]) * L.a2[i] * L.P));
+            L.a5[i] = log((exp(ev[i]) * L.a3[i] * L.P) / (exp(lnphi0[i]) * L.a2[i] * L.P));
             e2 += L.a5[i] * L.a5[i];
         }
         const double eNR = sqrt(e2);
@@ -620,7 +628,15 @@ __global__ void __launch_bounds__(256) pe_batched(const DevFluid* __restrict__ f
             }
         }
         if (!__any(have)) break;
-        if (have && L.st != ST_DONE) pe_step<NC>(*opaque(fp), io, L);
+        // Newton steps (matrix build + solve) cost several SSM steps; a lane
+        // that wants one waits until NR_BATCH lanes of the wavefront do (or
+        // nothing else is left to run), so the wavefront pays for the Newton
+        // branch once per batch instead of on nearly every step.
+        const bool live = have && L.st != ST_DONE;
+        const bool nr = live && (L.st == ST_T_NR || L.st == ST_S_NR);
+        const unsigned long long mnr = __ballot(nr), mlive = __ballot(live);
+        const bool run_nr = __popcll(mnr) >= NR_BATCH || mnr == mlive;
+        if (live && (!nr || run_nr)) pe_step<NC>(*opaque(fp), io, L);
     }
 }
 
