@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--assembly", type=str, default="100x100x50",
                    help="grid of the assembly measurement, '' to skip")
     p.add_argument("--modes", type=str, default="plain,batched")
+    p.add_argument("--adjoint", type=str, default="60x220x85",
+                   help="grid of the adjoint (J^T lambda) measurement, '' to skip")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only copy a buffer of this many MiB on the GPU (calibrates the "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -154,6 +156,46 @@ def assembly_measure(spec, steps, warmup, sync, dt):
                          "algorithmic_bytes": byts}}
 
 
+def adjoint_measure(spec, steps, sync, dt):
+    """Jᵀλ and ∂(λ·R)/∂(trans, Vp) after a Newton pass on the channelized grid."""
+    import torch
+    from opencaeporox_amd import workload as W
+    from opencaeporox_amd.fim import BATCHED, IsoTFIM
+    from opencaeporox_amd.roofline import PEAK_HBM_GBS
+    nx, ny, nz = (int(v) for v in spec.split("x"))
+    wl = W.config4(nx, ny, nz, Nc=4)
+    dev = IsoTFIM.from_workload(wl)
+    dev.UpdateLastTimeStep()
+    dev.autotune_flash()
+    dev.ResetToLastTimeStep()
+    nb, B, nconn = dev.nb, dev.bs, dev.nconn
+    lam = torch.randn(nb * B, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(lam)
+    dT = torch.empty(nconn, dtype=torch.float64, device="cuda")
+    dV = torch.empty(nb, dtype=torch.float64, device="cuda")
+    sync()
+    for it in range(steps + 1):
+        if it == 1:
+            dev.timing(True)
+            sync()
+            t0 = time.perf_counter()
+        dev.ResetToLastTimeStep()
+        dev.newton_pass(dt, BATCHED)
+        dev.jt_lambda_dev(lam.data_ptr(), out.data_ptr())
+        dev.dR_dparams_dev(dt, lam.data_ptr(), dT.data_ptr(), dV.data_ptr())
+    sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    k = {name: dev.timing_get(i)[0] for i, name in enumerate(("pe", "tail", "krpc_rock", "res", "asm", "jt", "dR"))}
+    nnzb = nb + 2 * nconn
+    jt_bytes = nnzb * (B * B * 8 + 4 + 4 + 4) + nb * 8 + 2 * nb * B * 8   # blocks, colind/tslot, rowptr, lambda, out
+    gbs = jt_bytes / (k["jt"] * 1e-3) / 1e9
+    dev.close()
+    return {"workload": wl.name, "cells": nb, "conns": nconn, "block": B, "ms_per_pass": ms,
+            "cells_per_s": nb / (ms * 1e-3), "kernel_ms": k,
+            "jt_roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes": jt_bytes}}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -227,6 +269,9 @@ def main():
     if a.assembly:
         line["assembly"] = assembly_measure(a.assembly, max(3, a.steps // 2), 1, sync, dt)
         log(f"assembly: {line['assembly']['kernel_ms']}")
+    if a.adjoint:
+        line["adjoint"] = adjoint_measure(a.adjoint, max(3, a.steps // 2), sync, dt)
+        log(f"adjoint: {line['adjoint']['kernel_ms']}")
     if a.cpu_cells > 0:
         line["cpu_baseline"] = cpu_baseline(wl, min(a.cpu_cells, wl.grid.nb), dt)
         log(f"cpu baseline: {line['cpu_baseline']}")

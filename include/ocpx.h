@@ -258,6 +258,13 @@ int  ocpx_update_last_step(ocpx_ctx* ctx);
  * OCPX_E_STATE before the first ocpx_update_last_step. */
 int  ocpx_reset_to_last_step(ocpx_ctx* ctx);
 
+/* Replace the bulk volumes [nb] and/or the connection transmissibilities
+ * [nconn] (host arrays; NULL keeps the current values).  The reference sets
+ * them once from the grid (Bulk::v, BulkConnTransMethod01::CalTrans,
+ * src/BulkConnFunc.cpp:15-67); here they are also the parameters of the
+ * adjoint gradients (ocpx_dR_dparams). */
+int  ocpx_set_geometry(ocpx_ctx* ctx, const double* v, const double* trans);
+
 /* One full Newton-step pass: flash_der + krpc + rock + res + assemble.
  * With use_graph != 0 the pass is captured once into a hipGraph and
  * replayed afterwards (dt is baked in at capture; re-captured if dt or
@@ -284,7 +291,7 @@ int  ocpx_dR_dparams_dev(ocpx_ctx* ctx, double dt, const double* lambda_d,
 /* Kernel timing: average duration (ms) of the kernels of one entry point
  * over the calls since the last reset, from HIP events recorded on the
  * context stream.  which: 0 PE, 1 flash tail, 2 krpc+rock, 3 residual,
- * 4 assembly. */
+ * 4 assembly, 5 J^T lambda, 6 dR/dparams. */
 int  ocpx_timing_enable(ocpx_ctx* ctx, int on);
 int  ocpx_timing_get(ocpx_ctx* ctx, int which, double* avg_ms, int64_t* calls);
 

@@ -114,6 +114,7 @@ def load(path=None):
         "ocpx_newton_pass": (C.c_int, [vp, dbl, C.c_int, C.c_int]),
         "ocpx_update_last_step": (C.c_int, [vp]),
         "ocpx_set_flash_mapping": (C.c_int, [vp, C.c_int]),
+        "ocpx_set_geometry": (C.c_int, [vp, vp, vp]),
         "ocpx_get_flash_mapping": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_autotune_flash": (C.c_int, [vp, vp]),
         "ocpx_reset_to_last_step": (C.c_int, [vp]),

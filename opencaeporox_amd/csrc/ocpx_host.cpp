@@ -80,7 +80,7 @@ struct ocpx_ctx {
     int team_blocks[2] = {0, 0}; // persistent grid of the team kernels, T = 8 / 16
     // timing
     bool timing = false;
-    TimingAcc tacc[5];
+    TimingAcc tacc[7];
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -789,6 +789,16 @@ int ocpx_reset_to_last_step(ocpx_ctx* c)
     return OCPX_OK;
 }
 
+int ocpx_set_geometry(ocpx_ctx* c, const double* v, const double* trans)
+{
+    if (!c) return OCPX_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (v) HIPCHK(c, hipMemcpy(c->v, v, (size_t)c->nb * 8, hipMemcpyHostToDevice));
+    if (trans && c->nconn) HIPCHK(c, hipMemcpy(c->trans, trans, (size_t)c->nconn * 8, hipMemcpyHostToDevice));
+    c->aux_valid = false;
+    return OCPX_OK;
+}
+
 int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
 {
     if (!c || (mode != OCPX_FLASH_PLAIN && mode != OCPX_FLASH_BATCHED)) return fail(c, OCPX_E_ARG, "bad mode");
@@ -831,6 +841,7 @@ int ocpx_jt_lambda_dev(ocpx_ctx* c, const double* lam, double* out)
 {
     if (!c || !lam || !out) return OCPX_E_ARG;
     if (!c->assembled) return fail(c, OCPX_E_STATE, "ocpx_jt_lambda before ocpx_assemble");
+    TimerScope t(c, 5);
     HIPCHK(c, run_jt_lambda(c->nc, c->nb, bsr_io(c), lam, out, c->st));
     return OCPX_OK;
 }
@@ -852,6 +863,7 @@ int ocpx_dR_dparams_dev(ocpx_ctx* c, double dt, const double* lam, double* dT, d
 {
     if (!c || !lam) return OCPX_E_ARG;
     if (!c->flashed) return fail(c, OCPX_E_STATE, "ocpx_dR_dparams before ocpx_flash_der");
+    TimerScope t(c, 6);
     HIPCHK(c, run_dR_dtrans(c->nc, cell_io(c), conn_io(c), dt, lam, dT ? dT : c->gT, dVp ? dVp : c->gV, c->st));
     return OCPX_OK;
 }
@@ -886,7 +898,7 @@ int ocpx_timing_enable(ocpx_ctx* c, int on)
 
 int ocpx_timing_get(ocpx_ctx* c, int which, double* avg_ms, int64_t* calls)
 {
-    if (!c || which < 0 || which > 4) return OCPX_E_ARG;
+    if (!c || which < 0 || which > 6) return OCPX_E_ARG;
     TimingAcc& t = c->tacc[which];
     for (auto& p : t.pending) {
         HIPCHK(c, hipEventSynchronize(p.second));

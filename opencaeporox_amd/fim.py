@@ -156,6 +156,16 @@ class IsoTFIM:
         u = np.ascontiguousarray(u, np.float64)
         self._chk(self.lib.ocpx_get_solution(self.h, u.ctypes.data, dSmax), "get solution")
 
+    def set_geometry(self, v=None, trans=None):
+        va = None if v is None else np.ascontiguousarray(v, np.float64)
+        ta = None if trans is None else np.ascontiguousarray(trans, np.float64)
+        if va is not None and va.size != self.nb:
+            raise ValueError("v must have one entry per cell")
+        if ta is not None and ta.size != self.nconn:
+            raise ValueError("trans must have one entry per connection")
+        self._chk(self.lib.ocpx_set_geometry(self.h, None if va is None else va.ctypes.data,
+                                             None if ta is None else ta.ctypes.data), "set geometry")
+
     def UpdateLastTimeStep(self):
         self._chk(self.lib.ocpx_update_last_step(self.h), "update last step")
 
@@ -179,6 +189,13 @@ class IsoTFIM:
         self._chk(self.lib.ocpx_dR_dparams(self.h, dt, lam.ctypes.data, dT.ctypes.data, dV.ctypes.data),
                   "dR/dparams")
         return dT[: self.nconn], dV
+
+    def jt_lambda_dev(self, lam_ptr, out_ptr):
+        """J^T lambda on device arrays (raw device pointers, [nb*(nc+1)])."""
+        self._chk(self.lib.ocpx_jt_lambda_dev(self.h, lam_ptr, out_ptr), "J^T lambda (device)")
+
+    def dR_dparams_dev(self, dt, lam_ptr, dT_ptr, dV_ptr):
+        self._chk(self.lib.ocpx_dR_dparams_dev(self.h, dt, lam_ptr, dT_ptr, dV_ptr), "dR/dparams (device)")
 
     # ---- timing
     def timing(self, on=True):

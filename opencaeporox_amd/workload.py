@@ -292,6 +292,43 @@ def config3(nx=100, ny=100, nz=50, Nc=6, seed=3):
     return Workload(f"C3 {nx}x{ny}x{nz} Nc={Nc}", fl, make_sat(), rock, grid, st)
 
 
+def channel_facies(nx, ny, nz, rng, nchan=6, width=0.035):
+    """Seeded binary facies: sinuous channels along x, one set per layer
+    drifting slowly with depth.  1 = channel."""
+    x = (np.arange(nx) + 0.5) / nx
+    y = (np.arange(ny) + 0.5) / ny
+    fac = np.zeros((nz, ny, nx), np.int8)
+    c0 = rng.uniform(0, 1, nchan)
+    amp = rng.uniform(0.03, 0.12, nchan)
+    wl = rng.uniform(0.3, 1.0, nchan)
+    ph = rng.uniform(0, 2 * np.pi, nchan)
+    for k in range(nz):
+        drift = 0.15 * np.sin(2 * np.pi * k / max(nz, 1) + ph)
+        for c in range(nchan):
+            yc = (c0[c] + drift[c] + amp[c] * np.sin(2 * np.pi * x / wl[c] + ph[c])) % 1.0
+            d = np.abs(y[:, None] - yc[None, :])
+            fac[k] |= (np.minimum(d, 1 - d) < width).astype(np.int8)
+    return fac.ravel()
+
+
+def config4(nx=60, ny=220, nz=85, Nc=4, seed=4, lbc_exact=0):
+    """Channelized facies (2000 / 1 mD means, log-normal sigma = 1 inside
+    each facies), Nc = 4, hydrostatic P + N(0, 1 MPa)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nb = nx * ny * nz
+    fac = channel_facies(nx, ny, nz, rng)
+    k = np.exp(np.where(fac == 1, np.log(2000.0), np.log(1.0)) + rng.normal(0, 1.0, nb))
+    poro = np.where(fac == 1, rng.uniform(0.2, 0.3, nb), rng.uniform(0.05, 0.15, nb))
+    grid = cartesian_grid(nx, ny, nz, 6.096, k, k, 0.1 * k, poro)
+    fl = make_fluid(Nc, seed + 100)
+    fl.lbc_exact = lbc_exact
+    rock = make_rock()
+    P = 200.0 + 0.0980665 * (grid.depth - grid.depth.min()) + rng.normal(0, 10.0, nb)
+    z = rng.dirichlet(np.ones(Nc), nb)
+    st = initial_state(fl, rock, grid, P, z, rng)
+    return Workload(f"C4 {nx}x{ny}x{nz} Nc={Nc} channelized", fl, make_sat(), rock, grid, st)
+
+
 def flash_cells(n, Nc, seed, P=None, alpha=0.5):
     """Independent cells (no connections) for the flash-throughput workload."""
     rng = np.random.Generator(np.random.PCG64(seed))
