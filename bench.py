@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--assembly", type=str, default="100x100x50",
                    help="grid of the assembly measurement, '' to skip")
     p.add_argument("--modes", type=str, default="plain,batched")
+    p.add_argument("--multimodel", type=int, default=0,
+                   help="models of the uneven multi-model batch (256 = the full batch; 0 = skip)")
     p.add_argument("--adjoint", type=str, default="60x220x85",
                    help="grid of the adjoint (J^T lambda) measurement, '' to skip")
     p.add_argument("--copy-mib", type=int, default=0,
@@ -196,6 +198,35 @@ def adjoint_measure(spec, steps, sync, dt):
                             "frac": gbs / PEAK_HBM_GBS, "algorithmic_bytes": jt_bytes}}
 
 
+def multimodel_measure(nmodels, steps, sync, dt):
+    """Uneven batch: cell counts log-uniform 5k-200k, Nc uniform in 3..12,
+    one context per Nc, each Newton step replayed from captured graphs."""
+    from opencaeporox_amd import workload as W
+    from opencaeporox_amd.fim import MultiModelBatch
+    t = time.perf_counter()
+    wls = W.config5(nmodels)
+    batch = MultiModelBatch(wls)
+    log(f"multi-model batch: {len(wls)} models, {batch.nb} cells, {len(batch.groups)} Nc groups, "
+        f"set up in {time.perf_counter() - t:.1f} s")
+    tune = batch.autotune()
+    batch.UpdateLastTimeStep()
+    for it in range(steps + 1):
+        if it == 1:
+            sync()
+            t0 = time.perf_counter()
+        batch.ResetToLastTimeStep()
+        batch.newton_pass(dt, graph=True)
+    sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    out = {"workload": f"{len(wls)} models, cells log-uniform 5k-200k, Nc uniform 3..12",
+           "models": len(wls), "cells": batch.nb, "conns": batch.nconn, "ms_per_step": ms,
+           "cells_per_s": batch.nb / (ms * 1e-3),
+           "groups": {int(Nc): {"models": len(idx), "cells": int(d.nb), "lanes_per_cell": d.flash_mapping(),
+                                "autotune_ms": tune[Nc]} for Nc, (d, idx) in batch.groups.items()}}
+    batch.close()
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -269,6 +300,9 @@ def main():
     if a.assembly:
         line["assembly"] = assembly_measure(a.assembly, max(3, a.steps // 2), 1, sync, dt)
         log(f"assembly: {line['assembly']['kernel_ms']}")
+    if a.multimodel:
+        line["multimodel"] = multimodel_measure(a.multimodel, 3, sync, dt)
+        log(f"multi-model: {line['multimodel']['cells_per_s']:.3e} cells/s")
     if a.adjoint:
         line["adjoint"] = adjoint_measure(a.adjoint, max(3, a.steps // 2), sync, dt)
         log(f"adjoint: {line['adjoint']['kernel_ms']}")

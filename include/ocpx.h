@@ -180,6 +180,17 @@ int64_t ocpx_field_len(int field, int32_t Nc);
 int  ocpx_ctx_create(const ocpx_fluid* fluid, const ocpx_sat* sat,
                      const ocpx_rock* rock, const ocpx_grid* grid,
                      void* hip_stream, ocpx_ctx** out);
+/* Several independent models of one component count Nc in one context (no
+ * counterpart in the reference, which runs one reservoir per process):
+ * fluid, sat, rock and grids are arrays of nmodel; the models' cells and
+ * connections are concatenated in model order (cell indices of model m are
+ * offset by the cells of models 0..m-1), each cell keeps its own model's
+ * fluid, tables and rock.  Fields, BSR and residuals are exchanged for the
+ * concatenated cells.  The flash runs in OCPX_FLASH_BATCHED mode with a team
+ * mapping. */
+int  ocpx_ctx_create_multi(int32_t nmodel, const ocpx_fluid* fluid, const ocpx_sat* sat,
+                           const ocpx_rock* rock, const ocpx_grid* grids, void* stream,
+                           ocpx_ctx** out);
 void ocpx_ctx_destroy(ocpx_ctx* ctx);
 const char* ocpx_last_error(const ocpx_ctx* ctx);
 int  ocpx_sync(ocpx_ctx* ctx);

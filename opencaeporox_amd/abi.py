@@ -94,6 +94,7 @@ def load(path=None):
         "ocpx_field_len": (i64, [C.c_int, i32]),
         "ocpx_ctx_create": (C.c_int, [C.POINTER(Fluid), C.POINTER(Sat), C.POINTER(Rock),
                                       C.POINTER(Grid), vp, C.POINTER(vp)]),
+        "ocpx_ctx_create_multi": (C.c_int, [i32, vp, vp, vp, vp, vp, C.POINTER(vp)]),
         "ocpx_ctx_destroy": (None, [vp]),
         "ocpx_last_error": (C.c_char_p, [vp]),
         "ocpx_sync": (C.c_int, [vp]),

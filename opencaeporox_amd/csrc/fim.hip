@@ -87,16 +87,18 @@ OCPX_HD void krpc_cell(const DevSat& s, double Sg, double Sw, double* kr, double
 // Connection aux record of a cell, per phase j at aux + (n*3 + j)*AUXJ:
 // [dlambda[b], dPj[b], drho[b], lambda], AUXJ = 3b + 1.
 template <int NCB>
-__global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, double Pref, double cp1,
-                                               double cp2, CellIO c)
+__global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, const DevRock* __restrict__ rp,
+                                               CellIO c)
 {
     constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, AUXJ = 3 * B + 1;
     const int lane = threadIdx.x, g = lane / B, r = lane % B;
     const int64_t n = (int64_t)blockIdx.x * G + g;
     if (g >= G || n >= c.nb) return;
     const size_t nb = c.nb;
+    const int model = c.mid ? c.mid[n] : 0;
+    const double Pref = rp[model].Pref, cp1 = rp[model].cp1, cp2 = rp[model].cp2;
     double kr[3], pc[3], dk[9], dp[9];
-    krpc_cell(*sp, c.S[nb + n], c.S[2 * nb + n], kr, pc, dk, dp);
+    krpc_cell(sp[model], c.S[nb + n], c.S[2 * nb + n], kr, pc, dk, dp);
     const double P = c.P[n];
     if (r == 0) {
         UNROLL for (int j = 0; j < 3; j++) {
@@ -408,13 +410,13 @@ hipError_t run_fill_u32(unsigned* p, unsigned v, int64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t run_krpc_rock(const DevSat* s, double Pref, double cp1, double cp2, const CellIO& c, hipStream_t st)
+hipError_t run_krpc_rock(const DevSat* s, const DevRock* r, const CellIO& c, hipStream_t st)
 {
     if (c.nb == 0) return hipSuccess;
     const int nc = c.nc;
     const int G = 64 / (nc + 1);
     const unsigned nblk = (unsigned)((c.nb + G - 1) / G);
-#define X_(N) case N: hipLaunchKernelGGL(krpc_aux<N>, dim3(nblk), dim3(64), 0, st, s, Pref, cp1, cp2, c); return hipGetLastError();
+#define X_(N) case N: hipLaunchKernelGGL(krpc_aux<N>, dim3(nblk), dim3(64), 0, st, s, r, c); return hipGetLastError();
     switch (nc) {
     X_(3) X_(4) X_(5) X_(6) X_(7) X_(8) X_(9) X_(10) X_(11) X_(12) X_(13)
     default: return hipErrorInvalidValue;

@@ -47,10 +47,12 @@ struct ocpx_ctx {
     hipStream_t st = nullptr;
     bool own_stream = false;
     std::string err;
-    DevFluid hf{};
-    DevFluid* df = nullptr;
-    DevSat* ds = nullptr;
-    ocpx_rock rock{};
+    DevFluid hf{};               // fluid of model 0 (host copy)
+    int nmodel = 1;
+    DevFluid* df = nullptr;      // [nmodel]
+    DevSat* ds = nullptr;        // [nmodel]
+    DevRock* drock = nullptr;    // [nmodel]
+    int32_t* mid = nullptr;      // [nb] model of each cell (nmodel > 1), else null
     Field f[OCPX_F_NFIELDS];
     Field lf[OCPX_F_NFIELDS];   // last-time-step copies (UpdateLastTimeStep)
     bool have_last = false;
@@ -185,6 +187,7 @@ CellIO cell_io(ocpx_ctx* c)
     io.poro = D(c, OCPX_F_PORO); io.poroP = D(c, OCPX_F_POROP); io.rockVp = D(c, OCPX_F_ROCKVP);
     io.v = c->v; io.depth = c->depth; io.poroInit = c->poroInit;
     io.aux = c->aux;
+    io.mid = c->mid;
     return io;
 }
 
@@ -240,6 +243,9 @@ int launch_pe(ocpx_ctx* c, int mode)
     io.queue = c->queue;
     if (mode == OCPX_FLASH_BATCHED) HIPCHK(c, run_fill_u32(c->queue, 0u, 1, c->st));
     TimerScope t(c, 0);
+    io.mid = c->mid;
+    if (c->mid && (mode != OCPX_FLASH_BATCHED || c->pe_lanes == 1))
+        return fail(c, OCPX_E_UNSUPPORTED, "contexts of several models flash in batched mode with a team mapping");
     if (mode == OCPX_FLASH_BATCHED && c->pe_lanes > 1) {
         const int T = c->pe_lanes;
         HIPCHK(c, run_pe_team(c->NC, T, c->df, io, true, c->team_blocks[T == 8 ? 0 : 1], c->st));
@@ -307,6 +313,7 @@ int launch_tail(ocpx_ctx* c)
     io.skip_minEig = D(c, OCPX_F_SKIP_MINEIG); io.skip_P = D(c, OCPX_F_SKIP_P);
     io.skip_T = D(c, OCPX_F_SKIP_T); io.skip_zi = D(c, OCPX_F_SKIP_ZI);
     TimerScope t(c, 1);
+    io.mid = c->mid;
     HIPCHK(c, run_tail(c->NC, c->df, io, c->st));
     return OCPX_OK;
 }
@@ -314,7 +321,7 @@ int launch_tail(ocpx_ctx* c)
 int launch_krpc_rock(ocpx_ctx* c)
 {
     TimerScope t(c, 2);
-    HIPCHK(c, run_krpc_rock(c->ds, c->rock.Pref, c->rock.cp1, c->rock.cp2, cell_io(c), c->st));
+    HIPCHK(c, run_krpc_rock(c->ds, c->drock, cell_io(c), c->st));
     c->aux_valid = c->flashed;
     return OCPX_OK;
 }
@@ -375,22 +382,31 @@ extern "C" {
 
 int64_t ocpx_field_len(int field, int32_t Nc) { return flen(field, Nc); }
 
-int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* rock, const ocpx_grid* g,
-                    void* stream, ocpx_ctx** out)
+}  // extern "C"
+
+namespace {
+// nmodel models share one context: fl, sat, rock are arrays of nmodel, g is
+// the concatenated grid and mid (nmodel > 1) the model of each cell.
+int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* rock,
+                    const ocpx_grid* g, const int32_t* mid, void* stream, ocpx_ctx** out)
 {
     if (!out) return OCPX_E_ARG;
     *out = nullptr;
-    if (!fl || !sat || !rock || !g) return OCPX_E_ARG;
-    if (fl->nc < 2 || fl->nc > 12) return OCPX_E_UNSUPPORTED;
+    if (!fl || !sat || !rock || !g || nmodel < 1) return OCPX_E_ARG;
+    for (int m = 0; m < nmodel; m++) {
+        if (fl[m].nc < 2 || fl[m].nc > 12) return OCPX_E_UNSUPPORTED;
+        if (fl[m].nc != fl[0].nc) return OCPX_E_ARG;
+        if (sat[m].nswof < 1 || sat[m].nswof > OCPX_MAX_TABROWS || sat[m].nsgof < 1 ||
+            sat[m].nsgof > OCPX_MAX_TABROWS)
+            return OCPX_E_ARG;
+    }
     if (g->nb <= 0 || g->nb > INT32_MAX || g->nconn < 0 || g->nconn > INT32_MAX) return OCPX_E_ARG;
-    if (sat->nswof < 1 || sat->nswof > OCPX_MAX_TABROWS || sat->nsgof < 1 || sat->nsgof > OCPX_MAX_TABROWS)
-        return OCPX_E_ARG;
     ocpx_ctx* c = new ocpx_ctx();
     c->NC = fl->nc;
     c->nc = fl->nc + 1;
     c->nb = g->nb;
     c->nconn = g->nconn;
-    c->rock = *rock;
+    c->nmodel = nmodel;
     if (stream) c->st = (hipStream_t)stream;
     else {
         if (hipStreamCreate(&c->st) != hipSuccess) { delete c; return OCPX_E_HIP; }
@@ -405,21 +421,38 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
             return OCPX_E_ARG;
         }
     }
-    // --- fluid constants (EoS_PR ctor, ViscosityMethod03 ctor, flash controls)
-    build_dev_fluid(fl, c->hf);
-    DevSat hs{};
-    hs.nswof = sat->nswof; hs.nsgof = sat->nsgof;
-    std::memcpy(hs.swof, sat->swof, sizeof(hs.swof));
-    std::memcpy(hs.sgof, sat->sgof, sizeof(hs.sgof));
+    // --- fluid constants (EoS_PR ctor, ViscosityMethod03 ctor, flash controls),
+    // saturation tables and rock, one set per model
+    std::vector<DevFluid> hfs(nmodel);
+    std::vector<DevSat> hss(nmodel);
+    std::vector<DevRock> hrs(nmodel);
+    for (int m = 0; m < nmodel; m++) {
+        build_dev_fluid(&fl[m], hfs[m]);
+        hss[m] = DevSat{};
+        hss[m].nswof = sat[m].nswof; hss[m].nsgof = sat[m].nsgof;
+        std::memcpy(hss[m].swof, sat[m].swof, sizeof(hss[m].swof));
+        std::memcpy(hss[m].sgof, sat[m].sgof, sizeof(hss[m].sgof));
+        hrs[m] = DevRock{rock[m].Pref, rock[m].cp1, rock[m].cp2};
+    }
+    c->hf = hfs[0];
 
     int rc;
 #define ALLOC(p, n) if ((rc = dalloc(c, &(p), (n))) != OCPX_OK) { *out = c; return rc; }
-    ALLOC(c->df, 1);
-    ALLOC(c->ds, 1);
-    if (hipMemcpy(c->df, &c->hf, sizeof(c->hf), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->ds, &hs, sizeof(hs), hipMemcpyHostToDevice) != hipSuccess) {
+    ALLOC(c->df, nmodel);
+    ALLOC(c->ds, nmodel);
+    ALLOC(c->drock, nmodel);
+    if (hipMemcpy(c->df, hfs.data(), sizeof(DevFluid) * nmodel, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->ds, hss.data(), sizeof(DevSat) * nmodel, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->drock, hrs.data(), sizeof(DevRock) * nmodel, hipMemcpyHostToDevice) != hipSuccess) {
         *out = c;
         return fail(c, OCPX_E_HIP, "copy of fluid constants failed");
+    }
+    if (nmodel > 1) {
+        ALLOC(c->mid, c->nb);
+        if (hipMemcpy(c->mid, mid, (size_t)c->nb * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            *out = c;
+            return fail(c, OCPX_E_HIP, "copy of the cell models failed");
+        }
     }
     for (int fi = 0; fi < OCPX_F_NFIELDS; fi++) {
         c->f[fi].len = (int)flen(fi, c->NC);
@@ -459,7 +492,8 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
     ALLOC(c->bmax, 2 * nblk); ALLOC(c->barg, 2 * nblk); ALLOC(c->norms4, 4);
     ALLOC(c->lam, (size_t)c->nb * B); ALLOC(c->out, (size_t)c->nb * B);
     ALLOC(c->gT, c->nconn); ALLOC(c->gV, c->nb);
-    const std::vector<int32_t> ord = pencil_order(c->nb, c->nconn, g->conn_b, g->conn_e);
+    const std::vector<int32_t> ord =
+        nmodel == 1 ? pencil_order(c->nb, c->nconn, g->conn_b, g->conn_e) : std::vector<int32_t>();
     if (!ord.empty()) ALLOC(c->order, c->nb);
 #undef ALLOC
     bool okc = hipMemcpy(c->rowptr, rp.data(), rp.size() * 8, hipMemcpyHostToDevice) == hipSuccess &&
@@ -496,8 +530,53 @@ int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* 
         c->team_blocks[q] = (ncu > 0 ? ncu : 256) * (b > 0 ? b : 4);
     }
     c->pe_lanes = default_lanes(c->NC);
+    if (nmodel > 1 && c->pe_lanes == 1) c->pe_lanes = 8;   // several fluids: team mapping
     *out = c;
     return OCPX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ocpx_ctx_create(const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* rock, const ocpx_grid* g,
+                    void* stream, ocpx_ctx** out)
+{
+    return ctx_create_impl(1, fl, sat, rock, g, nullptr, stream, out);
+}
+
+int ocpx_ctx_create_multi(int32_t nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* rock,
+                          const ocpx_grid* grids, void* stream, ocpx_ctx** out)
+{
+    if (!out) return OCPX_E_ARG;
+    *out = nullptr;
+    if (nmodel < 1 || !fl || !sat || !rock || !grids) return OCPX_E_ARG;
+    if (nmodel == 1) return ctx_create_impl(1, fl, sat, rock, grids, nullptr, stream, out);
+    int64_t nb = 0, nconn = 0;
+    for (int m = 0; m < nmodel; m++) {
+        if (grids[m].nb <= 0 || grids[m].nconn < 0) return OCPX_E_ARG;
+        nb += grids[m].nb;
+        nconn += grids[m].nconn;
+    }
+    if (nb > INT32_MAX || nconn > INT32_MAX) return OCPX_E_ARG;
+    std::vector<double> v(nb), depth(nb), poro(nb), trans(nconn);
+    std::vector<int32_t> cb(nconn), ce(nconn), mid(nb);
+    int64_t o = 0, k = 0;
+    for (int m = 0; m < nmodel; m++) {
+        const ocpx_grid& gm = grids[m];
+        std::memcpy(&v[o], gm.v, gm.nb * 8);
+        std::memcpy(&depth[o], gm.depth, gm.nb * 8);
+        std::memcpy(&poro[o], gm.poroInit, gm.nb * 8);
+        for (int64_t q = 0; q < gm.nb; q++) mid[o + q] = m;
+        for (int64_t q = 0; q < gm.nconn; q++) {
+            cb[k + q] = (int32_t)(gm.conn_b[q] + o);
+            ce[k + q] = (int32_t)(gm.conn_e[q] + o);
+            trans[k + q] = gm.trans[q];
+        }
+        o += gm.nb;
+        k += gm.nconn;
+    }
+    ocpx_grid g{nb, nconn, v.data(), depth.data(), poro.data(), cb.data(), ce.data(), trans.data()};
+    return ctx_create_impl(nmodel, fl, sat, rock, &g, mid.data(), stream, out);
 }
 
 void ocpx_ctx_destroy(ocpx_ctx* c)
@@ -580,7 +659,9 @@ int ocpx_set_flash_mapping(ocpx_ctx* c, int lanes)
 {
     if (!c) return OCPX_E_ARG;
     if (lanes == 0) lanes = c->pe_lanes_auto ? c->pe_lanes_auto : default_lanes(c->NC);
+    if (lanes == 1 && c->mid) lanes = 8;
     if (lanes != 1 && lanes != 8 && lanes != 16) return fail(c, OCPX_E_ARG, "lanes per cell must be 1, 8 or 16");
+    if (lanes == 1 && c->mid) return fail(c, OCPX_E_UNSUPPORTED, "contexts of several models need a team mapping");
     if (lanes > 1 && c->NC > lanes) return fail(c, OCPX_E_ARG, "a team of 8 lanes holds at most 8 components");
     if (lanes != c->pe_lanes) destroy_graph(c);
     c->pe_lanes = lanes;
@@ -610,6 +691,7 @@ int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
     int rc = OCPX_OK;
     for (int q = 0; q < 3 && !rc; q++) {
         if (cand[q] > 1 && c->NC > cand[q]) continue;
+        if (cand[q] == 1 && c->mid) continue;
         c->pe_lanes = cand[q];
         for (int rep = 0; rep < 2 && !rc; rep++) {      // first run warms caches and code
             hipEventRecord(a, c->st);

@@ -33,6 +33,7 @@ struct PeIO {
     int32_t* ftype_out;         // [nb]
     int32_t* iters;             // [8][nb] or null
     unsigned* queue;            // work-queue counter (batched mode)
+    const int32_t* mid;         // [nb] model of each cell (multi-model contexts), or null
 };
 
 // flash-derivative tail (tail.hip): properties, labels, dVf, dXsdXp,
@@ -52,6 +53,7 @@ struct TailIO {
     double *Nt, *vf, *vfP, *vfi, *dXsdXp;   // dXsdXp cell-major [nb][np*b][b]
     uint32_t* skip_flag;
     double *skip_minEig, *skip_P, *skip_T, *skip_zi;
+    const int32_t* mid;         // [nb] model of each cell, or null (one model)
 };
 
 // team mapping (pe_team.hip): T lanes per cell, T = 8 or 16
@@ -72,6 +74,12 @@ struct CellIO {
     double *kr, *Pc, *Pj, *dKrdS, *dPcdS, *poro, *poroP, *rockVp;
     const double *v, *depth, *poroInit;
     double* aux;                // [nb][3][3b+1] connection aux record (krpc_aux)
+    const int32_t* mid;         // [nb] model of each cell, or null (one model)
+};
+
+// rock compressibility of one model (OCPRockIsoT_Linear, src/OCPRock.cpp:19-24)
+struct DevRock {
+    double Pref, cp1, cp2;
 };
 
 // block-CSR and cell adjacency
@@ -91,8 +99,7 @@ struct ConnIO {
     const double* trans;
 };
 
-hipError_t run_krpc_rock(const DevSat* s, double rPref, double rcp1, double rcp2, const CellIO& c,
-                         hipStream_t st);
+hipError_t run_krpc_rock(const DevSat* s, const DevRock* r, const CellIO& c, hipStream_t st);
 hipError_t run_res_asm(int nc, const CellIO& c, const ConnIO& cn, const BsrIO& bsr, double dt,
                        bool do_res, bool do_jac, double* res, double* blockmax, int64_t* blockarg,
                        hipStream_t st);

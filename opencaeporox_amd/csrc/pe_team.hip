@@ -60,7 +60,7 @@ DEV double tmax(double v)
 template <int T>
 DEV double bc(double v, int k) { return __shfl(v, k, T); }
 
-// Lane constants of component t (fixed for the lane's lifetime).
+// Fluid constants of component t, loaded when the team takes a cell.
 template <int NC>
 struct LaneFluid {
     double Tc, Pc, logPc, mw, OA, OB, kwA;
@@ -78,7 +78,7 @@ DEV void lane_fluid(const DevFluid& f, int t, LaneFluid<NC>& lf)
 
 template <int NC, int T>
 struct Cell {
-    int cell, st, k, iter, maxIt, NP, ftype, flag, stage, restarts, ssm_conv, nr_conv, steps;
+    int cell, model, st, k, iter, maxIt, NP, ftype, flag, stage, restarts, ssm_conv, nr_conv, steps;
     uint32_t lNP;
     double P, Tk, logP, Yt, tol2, Sk, nu0, eNR0;
     int cnt[8];
@@ -374,12 +374,13 @@ DEV void relaxed_trial_end(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, 
 
 // ------------------------------------------------------------ begin / end
 template <int NC, int T>
-DEV void begin(const DevFluid& f, const LaneFluid<NC>& lf, const PeIO& io, int cell, Cell<NC, T>& C,
+DEV void begin(const DevFluid& f, LaneFluid<NC>& lf, const PeIO& io, int cell, Cell<NC, T>& C,
                bool act, int t)
 {
     const size_t n = io.nb;
     const int i = act ? t : 0;
     C.cell = cell;
+    lane_fluid<NC>(f, t, lf);
     C.P = io.P[cell];
     C.Tk = io.T[cell] + CONV4;
     C.logP = log(C.P);
@@ -690,7 +691,16 @@ DEV const DevFluid* opaque(const DevFluid* p)
     return p;
 }
 
-template <int NC, int T>
+// MULTI: cells of several models (io.mid), each with its own fluid; the
+// fluid pointer is then per team instead of uniform across the wavefront.
+template <int NC, int T, bool MULTI>
+DEV const DevFluid& fluid_of(const DevFluid* fp, const Cell<NC, T>& C)
+{
+    if constexpr (MULTI) return fp[C.model];
+    else return *opaque(fp);
+}
+
+template <int NC, int T, bool MULTI>
 __global__ void __launch_bounds__(64) pe_team_plain(const DevFluid* __restrict__ fp, PeIO io)
 {
     const int lane = __lane_id();
@@ -699,10 +709,10 @@ __global__ void __launch_bounds__(64) pe_team_plain(const DevFluid* __restrict__
     if (cell >= io.nb) return;                   // whole teams leave together
     const bool act = t < NC;
     LaneFluid<NC> lf;
-    lane_fluid<NC>(*fp, t, lf);
     Cell<NC, T> C;
-    begin<NC, T>(*fp, lf, io, cell, C, act, t);
-    while (C.st != ST_DONE) step<NC, T>(*opaque(fp), lf, io, C, act, t);
+    C.model = MULTI ? io.mid[cell] : 0;
+    begin<NC, T>(fp[C.model], lf, io, cell, C, act, t);
+    while (C.st != ST_DONE) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, act, t);
     end<NC, T>(io, C, act, t);
 }
 
@@ -710,7 +720,7 @@ __global__ void __launch_bounds__(64) pe_team_plain(const DevFluid* __restrict__
 // before the launch).  Only team leaders vote; a team whose cell is done
 // writes it back and takes the next index.  The wavefront leaves when the
 // queue is drained and no team has a cell.
-template <int NC, int T>
+template <int NC, int T, bool MULTI>
 __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict__ fp, PeIO io)
 {
     const int lane = __lane_id();
@@ -718,9 +728,9 @@ __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict
     const int lead = lane & ~(T - 1);
     const bool act = t < NC;
     LaneFluid<NC> lf;
-    lane_fluid<NC>(*fp, t, lf);
     Cell<NC, T> C;
     C.st = ST_DONE;
+    C.model = 0;
     bool have = false, drained = false;
     while (true) {
         if (C.st == ST_DONE && have) { end<NC, T>(io, C, act, t); have = false; }
@@ -733,12 +743,16 @@ __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict
             base = __shfl(base, first);
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lead) - 1ull));
-                if (idx < (unsigned)io.nb) { begin<NC, T>(*fp, lf, io, (int)idx, C, act, t); have = true; }
+                if (idx < (unsigned)io.nb) {
+                    C.model = MULTI ? io.mid[idx] : 0;
+                    begin<NC, T>(fp[C.model], lf, io, (int)idx, C, act, t);
+                    have = true;
+                }
                 else drained = true;
             }
         }
         if (!__any(have)) break;
-        if (have && C.st != ST_DONE) step<NC, T>(*opaque(fp), lf, io, C, act, t);
+        if (have && C.st != ST_DONE) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, act, t);
     }
 }
 
@@ -748,11 +762,14 @@ hipError_t launch(const DevFluid* f, const PeIO& io, bool batched, int grid_bloc
     if (io.nb == 0) return hipSuccess;
     const long lanes = io.nb * (long)T;
     long need = (lanes + 63) / 64;
+    const bool multi = io.mid != nullptr;
     if (!batched) {
-        hipLaunchKernelGGL((pe_team_plain<NC, T>), dim3((unsigned)need), dim3(64), 0, st, f, io);
+        if (multi) hipLaunchKernelGGL((pe_team_plain<NC, T, true>), dim3((unsigned)need), dim3(64), 0, st, f, io);
+        else hipLaunchKernelGGL((pe_team_plain<NC, T, false>), dim3((unsigned)need), dim3(64), 0, st, f, io);
     } else {
         const long nblk = grid_blocks < need ? grid_blocks : need;
-        hipLaunchKernelGGL((pe_team_batched<NC, T>), dim3((unsigned)nblk), dim3(64), 0, st, f, io);
+        if (multi) hipLaunchKernelGGL((pe_team_batched<NC, T, true>), dim3((unsigned)nblk), dim3(64), 0, st, f, io);
+        else hipLaunchKernelGGL((pe_team_batched<NC, T, false>), dim3((unsigned)nblk), dim3(64), 0, st, f, io);
     }
     return hipGetLastError();
 }
@@ -761,7 +778,7 @@ template <int NC, int T>
 int blocks_per_cu()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pe_team_batched<NC, T>, 64, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, pe_team_batched<NC, T, true>, 64, 0) != hipSuccess) return 0;
     return n;
 }
 

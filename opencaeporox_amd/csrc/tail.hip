@@ -138,7 +138,8 @@ __global__ void __launch_bounds__(64) flash_tail(const DevFluid* __restrict__ fp
 {
     const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= io.nb) return;
-    const DevFluid& f = *fp;
+    // one fluid: a uniform pointer; several models: the cell's own fluid
+    const DevFluid& f = io.mid ? fp[io.mid[cell]] : *fp;
     const size_t n = io.nb;
     constexpr int nc = NC + 1, np = 3, ncol = nc + 1, wC = NC;
     auto at = [&](int k) { return (size_t)k * n + cell; };

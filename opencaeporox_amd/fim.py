@@ -53,6 +53,38 @@ class IsoTFIM:
         obj.set_state(wl.state)
         return obj
 
+    @classmethod
+    def from_workloads(cls, wls, stream=None):
+        """Several models of one Nc in one context (ocpx_ctx_create_multi):
+        cells and connections concatenated in model order, each cell with its
+        own model's fluid, tables and rock."""
+        obj = cls.__new__(cls)
+        obj.lib = abi.load()
+        n = len(wls)
+        if n == 0 or any(w.fluid.nc != wls[0].fluid.nc for w in wls):
+            raise ValueError("models of one context must share Nc")
+        obj.fluid, obj.sat, obj.rock = wls[0].fluid, wls[0].sat, wls[0].rock
+        obj.Nc = wls[0].fluid.nc
+        obj.nc = obj.Nc + 1
+        obj.bs = obj.nc + 1
+        obj.nb = sum(w.grid.nb for w in wls)
+        obj.nconn = sum(w.grid.nconn for w in wls)
+        obj.offsets = np.cumsum([0] + [w.grid.nb for w in wls])
+        fl = (abi.Fluid * n)(*[w.fluid for w in wls])
+        sa = (abi.Sat * n)(*[w.sat for w in wls])
+        ro = (abi.Rock * n)(*[w.rock for w in wls])
+        gr = (abi.Grid * n)(*[w.grid.to_abi() for w in wls])
+        obj._keep = (fl, sa, ro, gr, [w.grid for w in wls])
+        h = C.c_void_p()
+        rc = obj.lib.ocpx_ctx_create_multi(n, fl, sa, ro, gr, stream, C.byref(h))
+        obj.h = h
+        if rc != 0:
+            msg = obj.lib.ocpx_last_error(h).decode() if h.value else ""
+            raise OcpxError(f"ocpx_ctx_create_multi failed ({rc}): {msg}")
+        for k in wls[0].state:
+            obj.set(k, np.concatenate([np.asarray(w.state[k]).ravel() for w in wls]))
+        return obj
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.ocpx_ctx_destroy(self.h)
@@ -206,3 +238,62 @@ class IsoTFIM:
         calls = C.c_int64()
         self._chk(self.lib.ocpx_timing_get(self.h, which, C.byref(ms), C.byref(calls)), "timing get")
         return ms.value, calls.value
+
+
+class MultiModelBatch:
+    """Independent models of uneven size and component count: one context per
+    Nc holding all models of that Nc (ocpx_ctx_create_multi), each context
+    with its own autotuned flash mapping and its own captured Newton step."""
+
+    def __init__(self, wls, stream=None):
+        groups = {}
+        for m, w in enumerate(wls):
+            groups.setdefault(w.fluid.nc, []).append(m)
+        self.models = wls
+        self.groups = {}          # Nc -> (context, model indices)
+        for Nc in sorted(groups):
+            idx = groups[Nc]
+            self.groups[Nc] = (IsoTFIM.from_workloads([wls[m] for m in idx], stream), idx)
+
+    @property
+    def nb(self):
+        return sum(d.nb for d, _ in self.groups.values())
+
+    @property
+    def nconn(self):
+        return sum(d.nconn for d, _ in self.groups.values())
+
+    def contexts(self):
+        return [d for d, _ in self.groups.values()]
+
+    def autotune(self):
+        return {Nc: d.autotune_flash() for Nc, (d, _) in self.groups.items()}
+
+    def UpdateLastTimeStep(self):
+        for d in self.contexts():
+            d.UpdateLastTimeStep()
+
+    def ResetToLastTimeStep(self):
+        for d in self.contexts():
+            d.ResetToLastTimeStep()
+
+    def newton_pass(self, dt, graph=False):
+        for d in self.contexts():
+            d.newton_pass(dt, BATCHED, graph=graph)
+
+    def sync(self):
+        for d in self.contexts():
+            d.sync()
+
+    def get(self, m, name):
+        """Field of model m (BulkVarSet layout)."""
+        Nc = self.models[m].fluid.nc
+        d, idx = self.groups[Nc]
+        k = idx.index(m)
+        L = abi.field_len(name, Nc)
+        a = d.get(name)
+        return a[d.offsets[k] * L:d.offsets[k + 1] * L]
+
+    def close(self):
+        for d in self.contexts():
+            d.close()

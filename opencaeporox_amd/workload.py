@@ -329,6 +329,36 @@ def config4(nx=60, ny=220, nz=85, Nc=4, seed=4, lbc_exact=0):
     return Workload(f"C4 {nx}x{ny}x{nz} Nc={Nc} channelized", fl, make_sat(), rock, grid, st)
 
 
+def model_box(ncell, nz=10):
+    """Cartesian box of about ncell cells: nz layers of a square plane."""
+    side = max(2, int(round(np.sqrt(ncell / nz))))
+    return side, side, nz
+
+
+def config5(nmodels=256, seed=5, cmin=5000, cmax=200000):
+    """Uneven multi-model batch: cell counts log-uniform in [cmin, cmax],
+    Nc uniform in {3..12}, each model with its own seeded PR fluid,
+    log-normal permeability, porosity and pressures."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for m in range(nmodels):
+        ncell = int(np.exp(rng.uniform(np.log(cmin), np.log(cmax))))
+        Nc = int(rng.integers(3, 13))
+        nx, ny, nz = model_box(ncell)
+        nb = nx * ny * nz
+        r = np.random.Generator(np.random.PCG64(seed * 100003 + m))
+        k = np.exp(r.normal(np.log(100.0), 1.0, nb))
+        poro = r.uniform(0.1, 0.3, nb)
+        grid = cartesian_grid(nx, ny, nz, 6.096, k, k, 0.1 * k, poro)
+        fl = make_fluid(Nc, seed * 1000 + m)
+        rock = make_rock()
+        P = r.uniform(100.0, 280.0, nb)
+        z = r.dirichlet(np.ones(Nc), nb)
+        st = initial_state(fl, rock, grid, P, z, r)
+        out.append(Workload(f"model {m} {nx}x{ny}x{nz} Nc={Nc}", fl, make_sat(), rock, grid, st))
+    return out
+
+
 def flash_cells(n, Nc, seed, P=None, alpha=0.5):
     """Independent cells (no connections) for the flash-throughput workload."""
     rng = np.random.Generator(np.random.PCG64(seed))
