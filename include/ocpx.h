@@ -220,15 +220,16 @@ int  ocpx_get_pe_result(ocpx_ctx* ctx, int32_t* NP, double* nu, double* x);
 /* Cell-to-lane mapping of the batched flash's phase-equilibrium kernel.  No
  * counterpart in the reference (its flash is a serial loop over cells,
  * src/IsoThermalMethod.cpp:1016-1071); same results either way.
- * lanes_per_cell: 1 = one cell per lane; 8 or 16 = one component per lane
- * in a team of that many lanes (8 needs Nc <= 8); 0 = default (the
- * autotuner's choice if it ran, else 1 for Nc <= 4 and the smallest team
- * that holds Nc above).  OCPX_FLASH_PLAIN is always one cell per lane. */
+ * lanes_per_cell: 1 = one cell per lane; 4, 8 or 16 = a team of that many
+ * lanes per cell, lane t holding components t, t+T, ... (4 for any Nc, 8 for
+ * Nc >= 5, 16 for Nc >= 9); 0 = default (the autotuner's choice if it ran,
+ * else 1 for Nc <= 4 and a team of 8 or 16 above; contexts of several models
+ * take 4 for Nc <= 4).  OCPX_FLASH_PLAIN is always one cell per lane. */
 int  ocpx_set_flash_mapping(ocpx_ctx* ctx, int lanes_per_cell);
 int  ocpx_get_flash_mapping(ocpx_ctx* ctx, int32_t* lanes_per_cell);
 /* Autotuner: runs the batched phase-equilibrium kernel on the current state
- * once per mapping that holds Nc (1, 8, 16 lanes per cell), keeps the
- * fastest as the context's mapping.  ms[3] (may be NULL) receives the kernel
+ * once per mapping built for Nc (1, 4, 8, 16 lanes per cell), keeps the
+ * fastest as the context's mapping.  ms[4] (may be NULL) receives the kernel
  * times in that order, 0 where a mapping does not apply.  Synchronous. */
 int  ocpx_autotune_flash(ocpx_ctx* ctx, double* ms);
 

@@ -77,9 +77,9 @@ struct ocpx_ctx {
     bool flashed = false, assembled = false;
     bool aux_valid = false;      // krpc_aux ran on the current flash state
     int grid_waves = 0;
-    int pe_lanes = 1;            // batched-flash mapping: lanes per cell (1, 8, 16)
+    int pe_lanes = 1;            // batched-flash mapping: lanes per cell (1, 4, 8, 16)
     int pe_lanes_auto = 0;       // the autotuner's choice, 0 if it has not run
-    int team_blocks[2] = {0, 0}; // persistent grid of the team kernels, T = 8 / 16
+    int team_blocks[3] = {0, 0, 0}; // persistent grid of the team kernels, T = 4 / 8 / 16
     // timing
     bool timing = false;
     TimingAcc tacc[7];
@@ -226,6 +226,8 @@ struct TimerScope {
     }
 };
 
+inline int team_slot(int T) { return T == 4 ? 0 : (T == 8 ? 1 : 2); }
+
 int launch_pe(ocpx_ctx* c, int mode)
 {
     PeIO io{};
@@ -248,14 +250,21 @@ int launch_pe(ocpx_ctx* c, int mode)
         return fail(c, OCPX_E_UNSUPPORTED, "contexts of several models flash in batched mode with a team mapping");
     if (mode == OCPX_FLASH_BATCHED && c->pe_lanes > 1) {
         const int T = c->pe_lanes;
-        HIPCHK(c, run_pe_team(c->NC, T, c->df, io, true, c->team_blocks[T == 8 ? 0 : 1], c->st));
+        HIPCHK(c, run_pe_team(c->NC, T, c->df, io, true, c->team_blocks[team_slot(T)], c->st));
     } else {
         HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
     }
     return OCPX_OK;
 }
 
-int default_lanes(int NC) { return NC <= 4 ? 1 : (NC <= 8 ? 8 : 16); }
+// Lanes per cell before the autotuner has run: one lane for a few components,
+// a team of 8 or 16 (one or two components per lane) above.  Contexts of
+// several models need a team (4 lanes below Nc = 5).
+int default_lanes(int NC, bool multi)
+{
+    if (NC <= 4) return multi ? 4 : 1;
+    return NC <= 8 ? 8 : 16;
+}
 
 // Row order for the assembly.  When the connections are those of a natural
 // Cartesian numbering (every e - b is 1, nx or nx*ny, x and y connections do
@@ -523,14 +532,13 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const int per_cu = pe_batched_blocks_per_cu(c->NC);
     c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
-    for (int q = 0; q < 2; q++) {
-        const int T = q == 0 ? 8 : 16;
-        if (c->NC > T) continue;
+    for (int q = 0; q < 3; q++) {
+        const int T = 4 << q;
+        if (!pe_team_size_ok(c->NC, T)) continue;
         const int b = pe_team_blocks_per_cu(c->NC, T);
         c->team_blocks[q] = (ncu > 0 ? ncu : 256) * (b > 0 ? b : 4);
     }
-    c->pe_lanes = default_lanes(c->NC);
-    if (nmodel > 1 && c->pe_lanes == 1) c->pe_lanes = 8;   // several fluids: team mapping
+    c->pe_lanes = default_lanes(c->NC, nmodel > 1);
     *out = c;
     return OCPX_OK;
 }
@@ -658,11 +666,12 @@ int ocpx_phase_equilibrium(ocpx_ctx* c, int mode)
 int ocpx_set_flash_mapping(ocpx_ctx* c, int lanes)
 {
     if (!c) return OCPX_E_ARG;
-    if (lanes == 0) lanes = c->pe_lanes_auto ? c->pe_lanes_auto : default_lanes(c->NC);
-    if (lanes == 1 && c->mid) lanes = 8;
-    if (lanes != 1 && lanes != 8 && lanes != 16) return fail(c, OCPX_E_ARG, "lanes per cell must be 1, 8 or 16");
+    if (lanes == 0) lanes = c->pe_lanes_auto ? c->pe_lanes_auto : default_lanes(c->NC, c->mid != nullptr);
+    if (lanes != 1 && lanes != 4 && lanes != 8 && lanes != 16)
+        return fail(c, OCPX_E_ARG, "lanes per cell must be 1, 4, 8 or 16");
     if (lanes == 1 && c->mid) return fail(c, OCPX_E_UNSUPPORTED, "contexts of several models need a team mapping");
-    if (lanes > 1 && c->NC > lanes) return fail(c, OCPX_E_ARG, "a team of 8 lanes holds at most 8 components");
+    if (lanes > 1 && !pe_team_size_ok(c->NC, lanes))
+        return fail(c, OCPX_E_ARG, "team size not built for this Nc (4: all; 8: Nc >= 5; 16: Nc >= 9)");
     if (lanes != c->pe_lanes) destroy_graph(c);
     c->pe_lanes = lanes;
     return OCPX_OK;
@@ -678,8 +687,8 @@ int ocpx_get_flash_mapping(ocpx_ctx* c, int32_t* lanes)
 int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
 {
     if (!c) return OCPX_E_ARG;
-    static const int cand[3] = {1, 8, 16};
-    double t[3] = {0, 0, 0};
+    static const int cand[4] = {1, 4, 8, 16};
+    double t[4] = {0, 0, 0, 0};
     const int keep = c->pe_lanes;
     const bool tm = c->timing;
     c->timing = false;
@@ -689,8 +698,8 @@ int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
     HIPCHK(c, hipEventCreate(&a));
     HIPCHK(c, hipEventCreate(&b));
     int rc = OCPX_OK;
-    for (int q = 0; q < 3 && !rc; q++) {
-        if (cand[q] > 1 && c->NC > cand[q]) continue;
+    for (int q = 0; q < 4 && !rc; q++) {
+        if (cand[q] > 1 && !pe_team_size_ok(c->NC, cand[q])) continue;
         if (cand[q] == 1 && c->mid) continue;
         c->pe_lanes = cand[q];
         for (int rep = 0; rep < 2 && !rc; rep++) {      // first run warms caches and code
@@ -710,7 +719,7 @@ int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
     c->timing = tm;
     c->pe_lanes = keep;
     if (rc) return rc;
-    if (ms) for (int q = 0; q < 3; q++) ms[q] = t[q];
+    if (ms) for (int q = 0; q < 4; q++) ms[q] = t[q];
     c->pe_lanes_auto = best;
     return ocpx_set_flash_mapping(c, best);
 }

@@ -56,7 +56,8 @@ struct TailIO {
     const int32_t* mid;         // [nb] model of each cell, or null (one model)
 };
 
-// team mapping (pe_team.hip): T lanes per cell, T = 8 or 16
+// team mapping (pe_team.hip): T lanes per cell, T = 4, 8 or 16
+bool pe_team_size_ok(int NC, int T);
 hipError_t run_pe_team(int NC, int T, const DevFluid* f, const PeIO& io, bool batched, int grid_blocks,
                        hipStream_t st);
 int pe_team_blocks_per_cu(int NC, int T);

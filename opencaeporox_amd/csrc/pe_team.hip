@@ -3,24 +3,29 @@
 // Same algorithm and state machine as pe.hip (OCPPhaseEquilibrium::
 // PhaseEquilibrium, src/OCPPhaseEquilibrium.cpp:71-112: stability by
 // StableSSM/StableNR or StableSSM01, split by SplitSSM2 + RachfordRice2 +
-// SplitNR + CheckSplit), mapped differently: lane t of a team owns component
-// t, so a cell's state is a handful of registers per lane instead of
-// ~7 Nc doubles in one lane.  That keeps Nc = 5..16 free of private-memory
-// spills and leaves room for several waves per SIMD.
+// SplitNR + CheckSplit), mapped differently: a team of T lanes holds one
+// cell and lane t owns components t, t+T, t+2T, ... (L = ceil(Nc/T) slots),
+// so a cell's state is a few registers per lane instead of ~7 Nc doubles in
+// one lane, and no Nc fills private memory.
 //
-//   * sums over components are butterfly reductions inside the team
-//     (every lane ends with the same bits, so control flow stays uniform
-//     across the team);
+//   * sums over components are butterfly reductions inside the team after a
+//     local sum over the lane's slots (every lane ends with the same bits, so
+//     control flow stays uniform across the team);
 //   * the mixture term s_i = sqrt(Ai) sum_k (1-k_ik) sqrt(Ak) x_k takes the
-//     other lanes' sqrt(Ak) x_k by shuffle;
-//   * Newton systems are held one row per lane and solved by Gaussian
-//     elimination with partial pivoting across the team (pivot search by
-//     butterfly argmax, pivot row by shuffle, column-oriented back
-//     substitution).
+//     other components' sqrt(Ak) x_k by shuffle, summed in k order;
+//   * Newton matrices are built row by row on the lanes owning the rows; the
+//     entries above the diagonal, which the reference takes from the lower
+//     triangle (dsysv_ 'U' on row-major storage), are evaluated on the row's
+//     own lane from the other component's broadcast quantities;
+//   * they are solved by Gaussian elimination with partial pivoting across
+//     the team (pivot search by butterfly argmax, pivot row by shuffle,
+//     column-oriented back substitution).
 //
-// A wavefront holds 64 / T teams.  pe_team_plain gives each team one cell;
-// pe_team_batched keeps teams resident and refills them from the global
-// work queue as their cells finish.
+// Scalar work per cell (cubic root, Rachford-Rice scalars) is shared by the
+// T lanes of a team, so a small T serves more cells per instruction; a large
+// T spreads the component work thinner.  The autotuner picks T per Nc.
+// pe_team_plain gives each team one cell; pe_team_batched keeps teams
+// resident and refills them from the global work queue as cells finish.
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 
@@ -60,45 +65,77 @@ DEV double tmax(double v)
 template <int T>
 DEV double bc(double v, int k) { return __shfl(v, k, T); }
 
-// Fluid constants of component t, loaded when the team takes a cell.
-template <int NC>
-struct LaneFluid {
-    double Tc, Pc, logPc, mw, OA, OB, kwA;
-    double omk[NC];          // 1 - k_tk
+// component k of a slot array: lane k % T, slot k / T (k compile-time)
+#define BCK(arr, k) bc<T>((arr)[(k) / T], (k) % T)
+#define SLOTS UNROLL for (int c = 0; c < L; c++)
+
+// slot s of an array, s uniform across the team but not compile-time
+template <int L>
+DEV double pick(const double (&v)[L], int s)
+{
+    double u = v[0];
+    UNROLL for (int c = 1; c < L; c++) u = (s == c) ? v[c] : u;
+    return u;
+}
+
+template <int NC, int T>
+struct Lane {
+    static constexpr int L = (NC + T - 1) / T;
+    int t;                 // lane within the team
+    bool act[L];           // slot c holds component c*T + t < NC
 };
 
-template <int NC>
-DEV void lane_fluid(const DevFluid& f, int t, LaneFluid<NC>& lf)
+// Fluid constants of the lane's components, loaded when the team takes a cell.
+template <int NC, int T>
+struct LaneFluid {
+    static constexpr int L = (NC + T - 1) / T;
+    double Tc[L], Pc[L], logPc[L], mw[L], OA[L], OB[L], kwA[L];
+    double omk[L][NC];     // 1 - k_ik
+};
+
+template <int NC, int T>
+DEV void lane_fluid(const DevFluid& f, const Lane<NC, T>& ln, LaneFluid<NC, T>& lf)
 {
-    const int i = t < NC ? t : 0;
-    lf.Tc = f.Tc[i]; lf.Pc = f.Pc[i]; lf.logPc = f.logPc[i]; lf.mw = f.mw[i];
-    lf.OA = f.OA[i]; lf.OB = f.OB[i]; lf.kwA = f.kwA[i];
-    UNROLL for (int k = 0; k < NC; k++) lf.omk[k] = f.omk[i * OCPX_NCMAX + k];
+    constexpr int L = LaneFluid<NC, T>::L;
+    SLOTS {
+        const int i = ln.act[c] ? c * T + ln.t : 0;
+        lf.Tc[c] = f.Tc[i]; lf.Pc[c] = f.Pc[i]; lf.logPc[c] = f.logPc[i]; lf.mw[c] = f.mw[i];
+        lf.OA[c] = f.OA[i]; lf.OB[c] = f.OB[i]; lf.kwA[c] = f.kwA[i];
+        UNROLL for (int k = 0; k < NC; k++) lf.omk[c][k] = f.omk[i * OCPX_NCMAX + k];
+    }
 }
 
 template <int NC, int T>
 struct Cell {
+    static constexpr int L = (NC + T - 1) / T;
     int cell, model, st, k, iter, maxIt, NP, ftype, flag, stage, restarts, ssm_conv, nr_conv, steps;
     uint32_t lNP;
     double P, Tk, logP, Yt, tol2, Sk, nu0, eNR0;
     int cnt[8];
-    // this lane's component
-    double z, y, a1, a2, a3, a4, a5;
-    double sA, Bi;              // sqrt(Ai), Bi at (P, T)
-    double s, A, B, Z;          // mixture at the last evaluated y (s of this component)
+    double z[L], y[L], a1[L], a2[L], a3[L], a4[L], a5[L];
+    double sA[L], Bi[L];        // sqrt(Ai), Bi at (P, T)
+    double s[L], A, B, Z;       // mixture at the last evaluated y
 };
+
+// masked team sum of a slot expression
+#define TSUM(expr) ([&]() { double acc_ = 0; SLOTS acc_ += ln.act[c] ? (expr) : 0.0; return tsum<T>(acc_); }())
 
 // ------------------------------------------------------------- EoS pieces
 template <int NC, int T>
-DEV void mix(const LaneFluid<NC>& lf, const Cell<NC, T>& C, bool act, double x, double& s,
-             double& A, double& B, double& Z)
+DEV void mix(const LaneFluid<NC, T>& lf, const Lane<NC, T>& ln, const Cell<NC, T>& C,
+             const double (&x)[Cell<NC, T>::L], double (&s)[Cell<NC, T>::L], double& A, double& B,
+             double& Z)
 {
-    const double w = act ? C.sA * x : 0.0;
-    double t = 0;
-    UNROLL for (int k = 0; k < NC; k++) t += lf.omk[k] * bc<T>(w, k);
-    s = C.sA * t;
-    A = tsum<T>(act ? x * s : 0.0);
-    B = tsum<T>(act ? C.Bi * x : 0.0);
+    constexpr int L = Cell<NC, T>::L;
+    double w[L], acc[L];
+    SLOTS { w[c] = ln.act[c] ? C.sA[c] * x[c] : 0.0; acc[c] = 0; }
+    UNROLL for (int k = 0; k < NC; k++) {
+        const double wk = BCK(w, k);
+        SLOTS acc[c] += lf.omk[c][k] * wk;
+    }
+    SLOTS s[c] = C.sA[c] * acc[c];
+    A = TSUM(x[c] * s[c]);
+    B = TSUM(C.Bi[c] * x[c]);
     Z = pr_z(A, B);
 }
 
@@ -121,8 +158,23 @@ DEV double zbcoef(double A, double B, double Z)
            (D1 + D2 - 1) * Z * Z;
 }
 
-// Entry (i,k) of d ln f_i / d n_k for one phase (src/OCPEoS.cpp:254-300);
-// i-quantities of this lane, k-quantities broadcast.
+// Entry (i,k) of d ln f_i / d x_k (src/OCPEoS.cpp:217-251): row i quantities
+// (Axi = 2 s_i, Bxi = B_i, x_i), column k quantities (Axk, Bxk, Zxk).
+DEV double lnfx_ik(double A, double B, double Z, double lG, double G, double ZB, double g2, double P,
+                   double Axi, double Bxi, double xi, double aik, bool diag, double Axk, double Bxk, double Zxk)
+{
+    const double C = xi * P / ZB;
+    const double E = -A / ((D1 - D2) * B) * (Axi / A - Bxi / B);
+    const double Cx = (ZB * (diag ? 1.0 : 0.0) - xi * (Zxk - Bxk)) * P / (ZB * ZB);
+    const double Dx = Bxi / B * (Zxk - Bxk * (Z - 1) / B);
+    double Ex = (2 * (A / B * Bxk * Bxi + B * aik) - Axi * Bxk - Axk * Bxi) / (B * B);
+    Ex /= -(D1 - D2);
+    const double Gx = g2 * (Z * Bxk - Zxk * B);
+    return 1 / C * Cx + Dx + Ex * lG + E / G * Gx;
+}
+
+// Entry (i,k) of d ln f_i / d n_k for one phase (src/OCPEoS.cpp:254-300):
+// row i quantities (s_i, B_i, x_i), column k quantities (B_k, An_k, Bn_k, Zn_k).
 DEV double lnfn_ik(double A, double B, double Z, double lG, double G, double P, double nt,
                    double si, double Bii, double xi, double aik, bool diag, double Bik, double Ank,
                    double Bnk, double Znk)
@@ -138,59 +190,60 @@ DEV double lnfn_ik(double A, double B, double Z, double lG, double G, double P, 
     return 1 / C * Cn + Dn + En * lG + E / G * Gn;
 }
 
-// Row t of a symmetric matrix whose lower triangle (entries k <= t) this lane
-// computed: entries k > t are taken from lane k's row, element t.
-template <int N, int T>
-DEV void mirror_lower(double (&M)[N], int t)
+// Solve M x = b with rows i = c*T + t on the team (partial pivoting, ties to
+// the lower row).  Returns x of the lane's components.
+template <int N, int T, int L>
+DEV void team_solve(double (&M)[L][N], double (&b)[L], const bool (&act)[L], int t, double (&sol)[L])
 {
-    double out[N];
-    UNROLL for (int k = 0; k < N; k++) out[k] = M[k];
-    UNROLL for (int d = 1; d < N; d++) {
-        double u = 0;
-        UNROLL for (int k = 0; k < N; k++) u = (k == t - d) ? M[k] : u;
-        const double v = bc<T>(u, t + d);
-        UNROLL for (int k = 0; k < N; k++) out[k] = (k == t + d) ? v : out[k];
-    }
-    UNROLL for (int k = 0; k < N; k++) M[k] = out[k];
-}
-
-// Solve M x = b, row r of M and b[r] on lane r < N (partial pivoting).
-// Returns x_t on lane t < N.
-template <int N, int T>
-DEV double team_solve(double (&M)[N], double b, int t)
-{
-    bool used = t >= N;
-    int ord = N;
+    bool used[L];
+    int ord[L];
+    SLOTS { used[c] = !act[c]; ord[c] = N; sol[c] = 0; }
     int piv[N];
-    UNROLL for (int c = 0; c < N; c++) {
-        double v = used ? -1.0 : fabs(M[c]);
-        v = (v == v) ? v : -0.5;                         // NaN rows pivot last
-        int p = t;
+    UNROLL for (int col = 0; col < N; col++) {
+        double v = -1.0;
+        int p = 1 << 30;
+        SLOTS {
+            double vc = used[c] ? -1.0 : fabs(M[c][col]);
+            vc = (vc == vc) ? vc : -0.5;                 // NaN rows pivot last
+            const int rc = c * T + t;
+            if (vc > v || (vc == v && rc < p)) { v = vc; p = rc; }
+        }
         UNROLL for (int o = T / 2; o > 0; o >>= 1) {
             const double v2 = __shfl_xor(v, o, T);
             const int p2 = __shfl_xor(p, o, T);
             if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
         }
-        piv[c] = p;
-        const double id = 1 / bc<T>(M[c], p);
-        const double bp = bc<T>(b, p);
+        piv[col] = p;
+        const int pl = p % T, ps = p / T;
+        double colv[L];
+        SLOTS colv[c] = M[c][col];
+        const double id = 1 / bc<T>(pick<L>(colv, ps), pl);
+        const double bp = bc<T>(pick<L>(b, ps), pl);
         double prow[N];
-        UNROLL for (int k = c + 1; k < N; k++) prow[k] = bc<T>(M[k], p);
-        if (t == p) { used = true; ord = c; }
-        else if (!used) {
-            const double l = M[c] * id;
-            UNROLL for (int k = c + 1; k < N; k++) M[k] -= l * prow[k];
-            b -= l * bp;
+        UNROLL for (int k = col + 1; k < N; k++) {
+            double mk[L];
+            SLOTS mk[c] = M[c][k];
+            prow[k] = bc<T>(pick<L>(mk, ps), pl);
+        }
+        SLOTS {
+            const bool mine = (t == pl) && (c == ps);
+            if (mine) { used[c] = true; ord[c] = col; }
+            else if (!used[c]) {
+                const double l = M[c][col] * id;
+                UNROLL for (int k = col + 1; k < N; k++) M[c][k] -= l * prow[k];
+                b[c] -= l * bp;
+            }
         }
     }
-    double sol = 0;
-    UNROLL for (int c = N - 1; c >= 0; c--) {
-        const int p = piv[c];
-        const double xc = bc<T>(b, p) / bc<T>(M[c], p);
-        if (t == c) sol = xc;
-        if (ord < c) b -= M[c] * xc;
+    UNROLL for (int col = N - 1; col >= 0; col--) {
+        const int p = piv[col];
+        const int pl = p % T, ps = p / T;
+        double colv[L];
+        SLOTS colv[c] = M[c][col];
+        const double xc = bc<T>(pick<L>(b, ps), pl) / bc<T>(pick<L>(colv, ps), pl);
+        if (t == col % T) sol[col / T] = xc;
+        SLOTS if (ord[c] < col) b[c] -= M[c][col] * xc;
     }
-    return sol;
 }
 
 // ------------------------------------------------------------ algorithm
@@ -199,32 +252,38 @@ DEV double trial_coef(int k)
     return k == 0 ? 1.0 : (k == 1 ? -1.0 : (k == 2 ? (1.0 / 3) : -(1.0 / 3)));
 }
 
-template <int NC>
-DEV double log_kw0(const LaneFluid<NC>& lf, double logP, double Tk)
+template <int NC, int T>
+DEV double log_kw0(const LaneFluid<NC, T>& lf, int c, double logP, double Tk)
 {
-    return lf.logPc - logP + lf.kwA * (1 - lf.Tc / Tk);
+    return lf.logPc[c] - logP + lf.kwA[c] * (1 - lf.Tc[c] / Tk);
 }
 
 template <int NC, int T>
-DEV void normalise_y(Cell<NC, T>& C, bool act)
+DEV void normalise_y(Cell<NC, T>& C, const Lane<NC, T>& ln)
 {
-    C.Yt = tsum<T>(act ? C.y : 0.0);
-    C.y *= 1 / C.Yt;
+    constexpr int L = Cell<NC, T>::L;
+    C.Yt = TSUM(C.y[c]);
+    const double r = 1 / C.Yt;
+    SLOTS C.y[c] *= r;
 }
 
 template <int NC, int T>
-DEV void rr2(const DevFluid& f, Cell<NC, T>& C, bool act)
+DEV void rr2(const DevFluid& f, Cell<NC, T>& C, const Lane<NC, T>& ln)
 {   // RachfordRice2 :651-705 on K = a4
-    const double Kmin = tmin<T>(act ? C.a4 : INFINITY);
-    const double Kmax = tmax<T>(act ? C.a4 : -INFINITY);
+    constexpr int L = Cell<NC, T>::L;
+    double kmn = INFINITY, kmx = -INFINITY;
+    SLOTS if (ln.act[c]) { kmn = fmin(kmn, C.a4[c]); kmx = fmax(kmx, C.a4[c]); }
+    const double Kmin = tmin<T>(kmn), Kmax = tmax<T>(kmx);
     const double numin = 1 / (1 - Kmax), numax = 1 / (1 - Kmin);
     double nu = 0.5 * (numin + numax);
     int it = 0;
-    const double km = C.a4 - 1;
+    double km[L];
+    SLOTS km[c] = C.a4[c] - 1;
     while (true) {
-        const double t = 1 / (1 + nu * km);
-        const double rj = tsum<T>(act ? C.z * km * t : 0.0);
-        const double J = -tsum<T>(act ? C.z * km * km * t * t : 0.0);
+        double tt[L];
+        SLOTS tt[c] = 1 / (1 + nu * km[c]);
+        const double rj = TSUM(C.z[c] * km[c] * tt[c]);
+        const double J = -TSUM(C.z[c] * km[c] * km[c] * tt[c] * tt[c]);
         if (fabs(rj) < f.RR_tol || it > f.RR_maxIt) break;
         const double d = -rj / J, tn = nu + d;
         if (tn < numax && tn > numin) nu = tn;
@@ -236,16 +295,20 @@ DEV void rr2(const DevFluid& f, Cell<NC, T>& C, bool act)
 }
 
 template <int NC, int T>
-DEV void update_x_rr(Cell<NC, T>& C)
+DEV void update_x_rr(Cell<NC, T>& C, const Lane<NC, T>& ln)
 {   // UpdateXRR :778-791
-    C.a3 = C.z / (1 + C.nu0 * (C.a4 - 1));
-    C.a2 = C.a4 * C.a3;
+    constexpr int L = Cell<NC, T>::L;
+    SLOTS {
+        C.a3[c] = ln.act[c] ? C.z[c] / (1 + C.nu0 * (C.a4[c] - 1)) : 0.0;
+        C.a2[c] = C.a4[c] * C.a3[c];
+    }
 }
 
 template <int NC, int T>
-DEV void split_start(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C, const PeIO& io,
-                     bool restart, int t)
+DEV void split_start(const DevFluid& f, const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const PeIO& io,
+                     bool restart, const Lane<NC, T>& ln)
 {   // SplitSSM2 :586-622
+    constexpr int L = Cell<NC, T>::L;
     C.ssm_conv = 1;
     C.tol2 = f.SSMsp_tol2;
     C.maxIt = f.SSMsp_maxIt;
@@ -253,10 +316,12 @@ DEV void split_start(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C,
         if (C.lNP == 2) {
             const size_t n = io.nb;
             const int nc = NC + 1;
-            const int i = t < NC ? t : 0;
-            C.a4 = io.xij[(size_t)i * n + C.cell] / io.xij[(size_t)(nc + i) * n + C.cell];
+            SLOTS {
+                const int i = ln.act[c] ? c * T + ln.t : 0;
+                C.a4[c] = io.xij[(size_t)i * n + C.cell] / io.xij[(size_t)(nc + i) * n + C.cell];
+            }
         } else {
-            C.a4 = exp(log_kw0(lf, C.logP, C.Tk));
+            SLOTS C.a4[c] = exp(log_kw0(lf, c, C.logP, C.Tk));
         }
     } else {
         C.tol2 *= 1E-1;
@@ -271,67 +336,72 @@ DEV void split_start(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C,
 }
 
 template <int NC, int T>
-DEV bool check_split(Cell<NC, T>& C, bool act)
+DEV bool check_split(Cell<NC, T>& C, const Lane<NC, T>& ln)
 {   // CheckSplit :960-1010
+    constexpr int L = Cell<NC, T>::L;
     if (C.NP != 2) return true;
     const double nu1 = 1 - C.nu0;
     const double nuMax = fmax(C.nu0, nu1);
-    const double eX = tsum<T>(act ? (C.a2 - C.a3) * (C.a2 - C.a3) : 0.0);
+    const double eX = TSUM((C.a2[c] - C.a3[c]) * (C.a2[c] - C.a3[c]));
     if (nuMax < 1 && C.nr_conv && finite_d(eX)) return true;
     if (!finite_d(eX) || 1 - nuMax < 1E-3) {
         C.NP = 1;
         C.nu0 = 1;
-        C.a2 = C.z;
+        SLOTS C.a2[c] = C.z[c];
         return false;
     }
     return true;
 }
 
 template <int NC, int T>
-DEV void finish_split(Cell<NC, T>& C, bool act)
+DEV void finish_split(Cell<NC, T>& C, const Lane<NC, T>& ln)
 {
-    check_split(C, act);
+    check_split(C, ln);
     C.st = ST_DONE;
 }
 
 template <int NC, int T>
-DEV void after_split_nr(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C, const PeIO& io,
-                        bool act, int t)
+DEV void after_split_nr(const DevFluid& f, const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const PeIO& io,
+                        const Lane<NC, T>& ln)
 {   // PhaseSplit restart loop :558-566
     if (C.stage == 0) {
-        if (C.nr_conv) { finish_split(C, act); return; }
+        if (C.nr_conv) { finish_split(C, ln); return; }
         C.stage = 1;
     } else {
-        if (!check_split(C, act)) { finish_split(C, act); return; }
-        if (C.ssm_conv || C.nr_conv) { finish_split(C, act); return; }
+        if (!check_split(C, ln)) { finish_split(C, ln); return; }
+        if (C.ssm_conv || C.nr_conv) { finish_split(C, ln); return; }
     }
-    if (++C.restarts > MAX_SPLIT_RESTARTS) { C.cnt[7] |= 0x40000000; finish_split(C, act); return; }
-    split_start(f, lf, C, io, true, t);
+    if (++C.restarts > MAX_SPLIT_RESTARTS) { C.cnt[7] |= 0x40000000; finish_split(C, ln); return; }
+    split_start(f, lf, C, io, true, ln);
 }
 
 template <int NC, int T>
-DEV void split_nr_begin(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C, const PeIO& io,
-                        double lnphi1, bool act, int t)
+DEV void split_nr_begin(const DevFluid& f, const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const PeIO& io,
+                        const double (&lnphi1)[Cell<NC, T>::L], const Lane<NC, T>& ln)
 {   // SplitNR :807-823; a5 = ln phi0 of x0
-    C.a1 = C.nu0 * C.a2;
-    C.a5 = log((exp(lnphi1) * C.a3 * C.P) / (exp(C.a5) * C.a2 * C.P));
-    const double eNR = sqrt(tsum<T>(act ? C.a5 * C.a5 : 0.0));
+    constexpr int L = Cell<NC, T>::L;
+    SLOTS {
+        C.a1[c] = C.nu0 * C.a2[c];
+        C.a5[c] = ln.act[c] ? log((exp(lnphi1[c]) * C.a3[c] * C.P) / (exp(C.a5[c]) * C.a2[c] * C.P)) : 0.0;
+    }
+    const double eNR = sqrt(TSUM(C.a5[c] * C.a5[c]));
     C.eNR0 = eNR;
     C.iter = 0;
     C.nr_conv = 0;
     if (eNR > f.NRsp_tol) { C.st = ST_S_NR; return; }
     C.nr_conv = eNR < f.NRsp_tol;
-    after_split_nr(f, lf, C, io, act, t);
+    after_split_nr(f, lf, C, io, ln);
 }
 
 template <int NC, int T>
-DEV void strict_decide(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C, const PeIO& io, int t)
+DEV void strict_decide(const DevFluid& f, const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const PeIO& io,
+                       const Lane<NC, T>& ln)
 {   // :460-470
     if (C.flag && C.Yt > 1 + f.eYt) {
         C.NP = 2;
         C.stage = 0;
         C.restarts = 0;
-        split_start(f, lf, C, io, false, t);
+        split_start(f, lf, C, io, false, ln);
         return;
     }
     if (++C.k < 4) { C.st = ST_T_START; return; }
@@ -339,19 +409,20 @@ DEV void strict_decide(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& 
 }
 
 template <int NC, int T>
-DEV void nr_sta_begin(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C, const PeIO& io,
-                      bool act, int t)
+DEV void nr_sta_begin(const DevFluid& f, const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const PeIO& io,
+                      const Lane<NC, T>& ln)
 {   // StableNR :478-488, resSTA in a3
-    const double e2 = tsum<T>(act ? C.a3 * C.a3 : 0.0);
+    constexpr int L = Cell<NC, T>::L;
+    const double e2 = TSUM(C.a3[c] * C.a3[c]);
     C.iter = 0;
     if (sqrt(e2) > f.NRsta_tol) { C.st = ST_T_NR; return; }
     C.flag = 1;
-    strict_decide(f, lf, C, io, t);
+    strict_decide(f, lf, C, io, ln);
 }
 
 template <int NC, int T>
-DEV void relaxed_trial_end(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, T>& C, const PeIO& io,
-                           int t)
+DEV void relaxed_trial_end(const DevFluid& f, const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const PeIO& io,
+                           const Lane<NC, T>& ln)
 {   // :377-392 and PhaseStable :283-291
     if (C.flag && C.Yt > 1 - 0.1 && C.Sk > 1) C.ftype = 2;
     bool stable;
@@ -369,24 +440,25 @@ DEV void relaxed_trial_end(const DevFluid& f, const LaneFluid<NC>& lf, Cell<NC, 
     C.NP = 2;
     C.stage = 0;
     C.restarts = 0;
-    split_start(f, lf, C, io, false, t);
+    split_start(f, lf, C, io, false, ln);
 }
 
 // ------------------------------------------------------------ begin / end
 template <int NC, int T>
-DEV void begin(const DevFluid& f, LaneFluid<NC>& lf, const PeIO& io, int cell, Cell<NC, T>& C,
-               bool act, int t)
+DEV void begin(const DevFluid& f, LaneFluid<NC, T>& lf, const PeIO& io, int cell, Cell<NC, T>& C,
+               const Lane<NC, T>& ln)
 {
+    constexpr int L = Cell<NC, T>::L;
     const size_t n = io.nb;
-    const int i = act ? t : 0;
     C.cell = cell;
-    lane_fluid<NC>(f, t, lf);
+    lane_fluid<NC, T>(f, ln, lf);
     C.P = io.P[cell];
     C.Tk = io.T[cell] + CONV4;
     C.logP = log(C.P);
-    const double Ni = act ? io.Ni[(size_t)i * n + cell] : 0.0;
-    const double Nt = tsum<T>(Ni);
-    C.z = act ? Ni / Nt : 0.0;
+    double Ni[L];
+    SLOTS Ni[c] = ln.act[c] ? io.Ni[(size_t)(c * T + ln.t) * n + cell] : 0.0;
+    const double Nt = TSUM(Ni[c]);
+    SLOTS C.z[c] = ln.act[c] ? Ni[c] / Nt : 0.0;
     uint32_t phaseNum = 0;
     UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
     C.lNP = phaseNum - 1u;
@@ -398,7 +470,9 @@ DEV void begin(const DevFluid& f, LaneFluid<NC>& lf, const PeIO& io, int cell, C
             if (io.skip_flag[cell]) {
                 const double me = io.skip_minEig[cell];
                 skip = fabs(1 - io.skip_P[cell] / C.P) < me / 10 && fabs(io.skip_T[cell] - C.Tk) < me * 10;
-                const bool mine = !act || fabs(Ni / Nt - io.skip_zi[(size_t)i * n + cell]) < me / 10;
+                bool mine = true;
+                SLOTS if (ln.act[c])
+                    mine = mine && fabs(Ni[c] / Nt - io.skip_zi[(size_t)(c * T + ln.t) * n + cell]) < me / 10;
                 skip = skip && tmin<T>(mine ? 1.0 : 0.0) > 0.5;
             }
             const uint32_t npPE = phaseNum - 1u;
@@ -410,14 +484,15 @@ DEV void begin(const DevFluid& f, LaneFluid<NC>& lf, const PeIO& io, int cell, C
         }
     }
     C.ftype = ft;
-    UNROLL for (int c = 0; c < 8; c++) C.cnt[c] = 0;
-    // EoS constants of this component (src/OCPEoS.cpp:68-84)
-    {
-        const double Pr = C.P / lf.Pc, Tr = C.Tk / lf.Tc;
-        const double q = 1 + lf.mw * (1 - sqrt(Tr));
-        const double Ai = lf.OA * Pr / (Tr * Tr) * (q * q);
-        C.sA = act ? sqrt(Ai) : 0.0;
-        C.Bi = act ? lf.OB * Pr / Tr : 0.0;
+    UNROLL for (int q = 0; q < 8; q++) C.cnt[q] = 0;
+    // EoS constants of the lane's components (src/OCPEoS.cpp:68-84)
+    SLOTS {
+        const double Pr = C.P / lf.Pc[c], Tr = C.Tk / lf.Tc[c];
+        const double q = 1 + lf.mw[c] * (1 - sqrt(Tr));
+        const double Ai = lf.OA[c] * Pr / (Tr * Tr) * (q * q);
+        C.sA[c] = ln.act[c] ? sqrt(Ai) : 0.0;
+        C.Bi[c] = ln.act[c] ? lf.OB[c] * Pr / Tr : 0.0;
+        C.a1[c] = C.a2[c] = C.a3[c] = C.a4[c] = C.a5[c] = 0;
     }
     C.NP = 1;
     C.nu0 = 1;
@@ -428,245 +503,308 @@ DEV void begin(const DevFluid& f, LaneFluid<NC>& lf, const PeIO& io, int cell, C
     C.Yt = 0;
     C.Sk = 0;
     C.steps = 0;
-    C.a1 = C.a2 = C.a3 = C.a4 = C.a5 = 0;
     if (ft == 1) { C.st = ST_DONE; return; }
     if (ft == 2) {
         C.NP = 2;
         C.Yt = 1.01;
-        split_start(f, lf, C, io, false, t);
+        split_start(f, lf, C, io, false, ln);
         return;
     }
-    C.y = C.z;
+    SLOTS C.y[c] = C.z[c];
     C.st = ST_FEED;
 }
 
 template <int NC, int T>
-DEV void end(const PeIO& io, const Cell<NC, T>& C, bool act, int t)
+DEV void end(const PeIO& io, const Cell<NC, T>& C, const Lane<NC, T>& ln)
 {
+    constexpr int L = Cell<NC, T>::L;
     const size_t n = io.nb;
-    const int c = C.cell;
+    const int cl = C.cell;
     const int NP = C.NP;
-    if (act) {
-        io.pe_x[(size_t)t * n + c] = NP > 1 ? C.a2 : C.z;
-        io.pe_x[(size_t)(NC + t) * n + c] = NP > 1 ? C.a3 : 0.0;
+    SLOTS if (ln.act[c]) {
+        const int i = c * T + ln.t;
+        io.pe_x[(size_t)i * n + cl] = NP > 1 ? C.a2[c] : C.z[c];
+        io.pe_x[(size_t)(NC + i) * n + cl] = NP > 1 ? C.a3[c] : 0.0;
     }
-    if (t == 0) {
-        io.pe_NP[c] = NP;
-        io.pe_nu[c] = NP > 1 ? C.nu0 : 1.0;
-        io.ftype_out[c] = NP > 1 ? 2 : C.ftype;            // :111
+    if (ln.t == 0) {
+        io.pe_NP[cl] = NP;
+        io.pe_nu[cl] = NP > 1 ? C.nu0 : 1.0;
+        io.ftype_out[cl] = NP > 1 ? 2 : C.ftype;            // :111
         if (io.iters)
-            UNROLL for (int q = 0; q < 8; q++) io.iters[(size_t)q * n + c] = C.cnt[q];
+            UNROLL for (int q = 0; q < 8; q++) io.iters[(size_t)q * n + cl] = C.cnt[q];
+    }
+}
+
+// ------------------------------------------------------------ Newton steps
+// StableNR (:488-516): Jacobian JmatSTA (:523-541), solved as the symmetric
+// matrix of its lower triangle (dsysv_ 'U' on row-major storage, :493).
+template <int NC, int T>
+DEV void stable_nr_pre(const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const Lane<NC, T>& ln)
+{
+    constexpr int L = Cell<NC, T>::L;
+    const double A = C.A, B = C.B, Z = C.Z, P = C.P;
+    const double den = zden(A, B, Z), bcf = zbcoef(A, B, Z);
+    const double G = (Z + D1 * B) / (Z + D2 * B), lG = log(G);
+    const double ZB = Z - B;
+    const double g2 = (D1 - D2) / (Z + D2 * B) / (Z + D2 * B);
+    double Zx[L], Ax[L];
+    SLOTS { Zx[c] = ((B - Z) * (2 * C.s[c]) + bcf * C.Bi[c]) / den; Ax[c] = 2 * C.s[c]; }
+    // own rows: entries (i, k) for every k and the row sums t_i = sum_k y_k S_ik
+    double row[L][NC], tr[L];
+    SLOTS tr[c] = 0;
+    UNROLL for (int k = 0; k < NC; k++) {
+        const double sAk = BCK(C.sA, k), Bxk = BCK(C.Bi, k), Axk = BCK(Ax, k);
+        const double Zxk = BCK(Zx, k), yk = BCK(C.y, k);
+        SLOTS {
+            const double aik = lf.omk[c][k] * C.sA[c] * sAk;
+            row[c][k] = lnfx_ik(A, B, Z, lG, G, ZB, g2, P, Ax[c], C.Bi[c], C.y[c], aik, c * T + ln.t == k,
+                                Axk, Bxk, Zxk);
+            tr[c] += yk * row[c][k];
+        }
+    }
+    // entries above the diagonal come from row k's lower triangle:
+    // S_ki = (lnfx(k, i) - t_k + 1) / Yt, evaluated here from k's quantities
+    double M[L][NC];
+    UNROLL for (int k = 0; k < NC; k++) {
+        const double sAk = BCK(C.sA, k), Bxk = BCK(C.Bi, k), Axk = BCK(Ax, k), yk = BCK(C.y, k);
+        const double tk = BCK(tr, k);
+        SLOTS {
+            const int i = c * T + ln.t;
+            const double own = (row[c][k] - tr[c] + 1) / C.Yt;
+            const double aki = lf.omk[c][k] * sAk * C.sA[c];
+            const double other = (lnfx_ik(A, B, Z, lG, G, ZB, g2, P, Axk, Bxk, yk, aki, false, Ax[c], C.Bi[c],
+                                          Zx[c]) - tk + 1) / C.Yt;
+            M[c][k] = (k <= i) ? own : other;
+        }
+    }
+    double rhs[L], sol[L];
+    SLOTS rhs[c] = C.a3[c];
+    team_solve<NC, T, L>(M, rhs, ln.act, ln.t, sol);
+    C.cnt[6]++;
+    C.cnt[7]++;
+    SLOTS C.y[c] = ln.act[c] ? C.y[c] * C.Yt + sol[c] : 0.0;
+    normalise_y(C, ln);
+}
+
+// SplitNR (:826-868): Jacobian of both phases' d ln f / d n (:839-854),
+// symmetric from its lower triangle (dsysv_ 'U', :835), CalStepNRsp (:939-957).
+template <int NC, int T>
+DEV void split_nr_pre(const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const Lane<NC, T>& ln)
+{
+    constexpr int L = Cell<NC, T>::L;
+    const double nu0 = C.nu0, nu1 = 1 - C.nu0, P = C.P;
+    double s0[L], s1[L], A0, B0, Z0, A1, B1, Z1;
+    mix<NC, T>(lf, ln, C, C.a2, s0, A0, B0, Z0);
+    mix<NC, T>(lf, ln, C, C.a3, s1, A1, B1, Z1);
+    const double den0 = zden(A0, B0, Z0), bc0 = zbcoef(A0, B0, Z0);
+    const double den1 = zden(A1, B1, Z1), bc1 = zbcoef(A1, B1, Z1);
+    const double G0 = (Z0 + D1 * B0) / (Z0 + D2 * B0), lG0 = log(G0);
+    const double G1 = (Z1 + D1 * B1) / (Z1 + D2 * B1), lG1 = log(G1);
+    double An0[L], Bn0[L], Zn0[L], An1[L], Bn1[L], Zn1[L];
+    SLOTS {   // src/OCPEoS.cpp:153-168
+        An0[c] = 2 / nu0 * (s0[c] - A0); Bn0[c] = 1 / nu0 * (C.Bi[c] - B0);
+        Zn0[c] = ((B0 - Z0) * An0[c] + bc0 * Bn0[c]) / den0;
+        An1[c] = 2 / nu1 * (s1[c] - A1); Bn1[c] = 1 / nu1 * (C.Bi[c] - B1);
+        Zn1[c] = ((B1 - Z1) * An1[c] + bc1 * Bn1[c]) / den1;
+    }
+    double J[L][NC];
+    UNROLL for (int k = 0; k < NC; k++) {
+        const double sAk = BCK(C.sA, k), Bik = BCK(C.Bi, k);
+        const double An0k = BCK(An0, k), Bn0k = BCK(Bn0, k), Zn0k = BCK(Zn0, k);
+        const double An1k = BCK(An1, k), Bn1k = BCK(Bn1, k), Zn1k = BCK(Zn1, k);
+        const double s0k = BCK(s0, k), s1k = BCK(s1, k), x0k = BCK(C.a2, k), x1k = BCK(C.a3, k);
+        SLOTS {
+            const int i = c * T + ln.t;
+            const double aik = lf.omk[c][k] * C.sA[c] * sAk;
+            const double aki = lf.omk[c][k] * sAk * C.sA[c];
+            // own row (i, k) for k <= i; row k's entry (k, i) above the diagonal
+            const double own = lnfn_ik(A0, B0, Z0, lG0, G0, P, nu0, s0[c], C.Bi[c], C.a2[c], aik, i == k, Bik,
+                                       An0k, Bn0k, Zn0k) +
+                               lnfn_ik(A1, B1, Z1, lG1, G1, P, nu1, s1[c], C.Bi[c], C.a3[c], aik, i == k, Bik,
+                                       An1k, Bn1k, Zn1k);
+            const double other = lnfn_ik(A0, B0, Z0, lG0, G0, P, nu0, s0k, Bik, x0k, aki, false, C.Bi[c],
+                                         An0[c], Bn0[c], Zn0[c]) +
+                                 lnfn_ik(A1, B1, Z1, lG1, G1, P, nu1, s1k, Bik, x1k, aki, false, C.Bi[c],
+                                         An1[c], Bn1[c], Zn1[c]);
+            J[c][k] = (k <= i) ? own : other;
+        }
+    }
+    C.cnt[6] += 2;
+    C.cnt[7]++;
+    double rhs[L], dn[L];
+    SLOTS rhs[c] = C.a5[c];
+    team_solve<NC, T, L>(J, rhs, ln.act, ln.t, dn);
+    double alpha = 1;
+    UNROLL for (int i = 0; i < NC; i++) {       // components in order
+        const double a1i = BCK(C.a1, i), dni = BCK(dn, i);
+        if (a1i + alpha * dni < 0) alpha *= 0.9 * fabs(a1i / dni);
+    }
+    double d[L];
+    SLOTS {
+        d[c] = alpha * dn[c];
+        C.a1[c] = ln.act[c] ? C.a1[c] + d[c] : 0.0;
+    }
+    C.nu0 = TSUM(C.a1[c]);
+    C.Sk = TSUM(2 * d[c] * d[c]);
+    const double r0 = 1 / C.nu0, r1 = 1 / (1 - C.nu0);
+    SLOTS {
+        C.a2[c] = ln.act[c] ? fabs(C.a1[c] * r0) : 0.0;
+        C.a3[c] = ln.act[c] ? fabs((C.z[c] - C.a1[c]) * r1) : 0.0;
+        C.y[c] = C.a2[c];
     }
 }
 
 // ------------------------------------------------------------ one step
 template <int NC, int T>
-DEV void step(const DevFluid& f, const LaneFluid<NC>& lf, const PeIO& io, Cell<NC, T>& C, bool act, int t)
+DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cell<NC, T>& C,
+              const Lane<NC, T>& ln)
 {
+    constexpr int L = Cell<NC, T>::L;
     if (++C.steps > MAX_STEPS) { C.cnt[7] |= 0x20000000; C.st = ST_DONE; return; }
     bool eval = true;
     switch (C.st) {
     case ST_T_START: {
         const double ck = trial_coef(C.k);
-        C.y = act ? C.z / exp(ck * log_kw0(lf, C.logP, C.Tk)) : 0.0;
-        normalise_y(C, act);
+        SLOTS C.y[c] = ln.act[c] ? C.z[c] / exp(ck * log_kw0(lf, c, C.logP, C.Tk)) : 0.0;
+        normalise_y(C, ln);
         break;
     }
     case ST_T_SSM:
-        C.y = act ? C.a1 / C.a2 : 0.0;
-        normalise_y(C, act);
+        SLOTS C.y[c] = ln.act[c] ? C.a1[c] / C.a2[c] : 0.0;
+        normalise_y(C, ln);
         break;
     case ST_A_ITER: {
-        C.y = C.z * C.a4;
-        normalise_y(C, act);
-        const double dY = tsum<T>(act ? (C.y - C.a3) * (C.y - C.a3) : 0.0);
+        SLOTS C.y[c] = C.z[c] * C.a4[c];
+        normalise_y(C, ln);
+        const double dY = TSUM((C.y[c] - C.a3[c]) * (C.y[c] - C.a3[c]));
         if (dY < DYTOL) { C.flag = 1; eval = false; }
         break;
     }
     case ST_A_START: {
         const double ck = trial_coef(C.k);
-        C.a4 = act ? exp(ck * log_kw0(lf, C.logP, C.Tk)) : 0.0;
-        C.y = C.z * C.a4;
-        normalise_y(C, act);
+        SLOTS {
+            C.a4[c] = ln.act[c] ? exp(ck * log_kw0(lf, c, C.logP, C.Tk)) : 0.0;
+            C.y[c] = C.z[c] * C.a4[c];
+        }
+        normalise_y(C, ln);
         C.iter = 0;
         C.flag = 0;
         break;
     }
-    case ST_T_NR: {   // Jacobian :523-541, symmetric from its lower triangle (dsysv_ 'U', :493)
-        const double A = C.A, B = C.B, Z = C.Z, P = C.P;
-        const double den = zden(A, B, Z), bcf = zbcoef(A, B, Z);
-        const double Zx = ((B - Z) * (2 * C.s) + bcf * C.Bi) / den;
-        const double G = (Z + D1 * B) / (Z + D2 * B), lG = log(G);
-        const double ZB = Z - B;
-        const double Axi = 2 * C.s, Bxi = C.Bi;
-        const double Cc = C.y * P / ZB;
-        const double E = -A / ((D1 - D2) * B) * (Axi / A - Bxi / B);
-        const double g2 = (D1 - D2) / (Z + D2 * B) / (Z + D2 * B);
-        double row[NC];
-        double tr = 0;
-        UNROLL for (int k = 0; k < NC; k++) {
-            const double sAk = bc<T>(C.sA, k), Bxk = bc<T>(C.Bi, k), Axk = 2 * bc<T>(C.s, k);
-            const double Zxk = bc<T>(Zx, k), yk = bc<T>(C.y, k);
-            const double aik = lf.omk[k] * C.sA * sAk;
-            const double Cx = (ZB * (t == k ? 1.0 : 0.0) - C.y * (Zxk - Bxk)) * P / (ZB * ZB);
-            const double Dx = Bxi / B * (Zxk - Bxk * (Z - 1) / B);
-            double Ex = (2 * (A / B * Bxk * Bxi + B * aik) - Axi * Bxk - Axk * C.Bi) / (B * B);
-            Ex /= -(D1 - D2);
-            const double Gx = g2 * (Z * Bxk - Zxk * B);
-            row[k] = 1 / Cc * Cx + Dx + Ex * lG + E / G * Gx;
-            tr += yk * row[k];
-        }
-        UNROLL for (int k = 0; k < NC; k++) row[k] = (row[k] - tr + 1) / C.Yt;
-        mirror_lower<NC, T>(row, t);
-        const double sol = team_solve<NC, T>(row, C.a3, t);
-        C.cnt[6]++;
-        C.cnt[7]++;
-        C.y = act ? C.y * C.Yt + sol : 0.0;
-        normalise_y(C, act);
+    case ST_T_NR:
+        stable_nr_pre(lf, C, ln);
         break;
-    }
     case ST_S_SSM_A:
-        rr2(f, C, act);
-        update_x_rr(C);
-        C.y = C.a2;
+        rr2(f, C, ln);
+        update_x_rr(C, ln);
+        SLOTS C.y[c] = C.a2[c];
         break;
     case ST_S_SSM_B:
-        C.y = C.a3;
+        SLOTS C.y[c] = C.a3[c];
         break;
-    case ST_S_NR: {   // :826-868
-        const double nu0 = C.nu0, nu1 = 1 - C.nu0, P = C.P;
-        double s0, A0, B0, Z0, s1, A1, B1, Z1;
-        mix<NC, T>(lf, C, act, C.a2, s0, A0, B0, Z0);
-        mix<NC, T>(lf, C, act, C.a3, s1, A1, B1, Z1);
-        // d ln f / d n precomputations of both phases (src/OCPEoS.cpp:153-168)
-        const double den0 = zden(A0, B0, Z0), bc0 = zbcoef(A0, B0, Z0);
-        const double den1 = zden(A1, B1, Z1), bc1 = zbcoef(A1, B1, Z1);
-        const double An0 = 2 / nu0 * (s0 - A0), Bn0 = 1 / nu0 * (C.Bi - B0);
-        const double Zn0 = ((B0 - Z0) * An0 + bc0 * Bn0) / den0;
-        const double An1 = 2 / nu1 * (s1 - A1), Bn1 = 1 / nu1 * (C.Bi - B1);
-        const double Zn1 = ((B1 - Z1) * An1 + bc1 * Bn1) / den1;
-        const double G0 = (Z0 + D1 * B0) / (Z0 + D2 * B0), lG0 = log(G0);
-        const double G1 = (Z1 + D1 * B1) / (Z1 + D2 * B1), lG1 = log(G1);
-        double J[NC];
-        UNROLL for (int k = 0; k < NC; k++) {
-            const double sAk = bc<T>(C.sA, k), Bik = bc<T>(C.Bi, k);
-            const double aik = lf.omk[k] * C.sA * sAk;
-            const double v0 = lnfn_ik(A0, B0, Z0, lG0, G0, P, nu0, s0, C.Bi, C.a2, aik, t == k, Bik,
-                                      bc<T>(An0, k), bc<T>(Bn0, k), bc<T>(Zn0, k));
-            const double v1 = lnfn_ik(A1, B1, Z1, lG1, G1, P, nu1, s1, C.Bi, C.a3, aik, t == k, Bik,
-                                      bc<T>(An1, k), bc<T>(Bn1, k), bc<T>(Zn1, k));
-            J[k] = v0 + v1;
-        }
-        mirror_lower<NC, T>(J, t);
-        C.cnt[6] += 2;
-        C.cnt[7]++;
-        const double dn = team_solve<NC, T>(J, C.a5, t);
-        // CalStepNRsp :939-957: alpha shrinks over the components in order
-        double alpha = 1;
-        UNROLL for (int i = 0; i < NC; i++) {
-            const double a1i = bc<T>(C.a1, i), dni = bc<T>(dn, i);
-            if (a1i + alpha * dni < 0) alpha *= 0.9 * fabs(a1i / dni);
-        }
-        const double d = alpha * dn;
-        C.a1 = act ? C.a1 + d : 0.0;
-        C.nu0 = tsum<T>(C.a1);
-        C.Sk = tsum<T>(act ? 2 * d * d : 0.0);
-        const double r0 = 1 / C.nu0, r1 = 1 / (1 - C.nu0);
-        C.a2 = act ? fabs(C.a1 * r0) : 0.0;
-        C.a3 = act ? fabs((C.z - C.a1) * r1) : 0.0;
-        C.y = C.a2;
+    case ST_S_NR:
+        split_nr_pre(lf, C, ln);
         break;
-    }
     default:   // ST_FEED: y = z
         break;
     }
 
     // common evaluator: ln phi at y
-    double ev = 0;
+    double ev[L];
+    SLOTS ev[c] = 0;
     if (eval) {
-        mix<NC, T>(lf, C, act, C.y, C.s, C.A, C.B, C.Z);
-        ev = act ? lnphi_of(C.s, C.Bi, C.A, C.B, C.Z) : 0.0;
+        mix<NC, T>(lf, ln, C, C.y, C.s, C.A, C.B, C.Z);
+        SLOTS ev[c] = ln.act[c] ? lnphi_of(C.s[c], C.Bi[c], C.A, C.B, C.Z) : 0.0;
         C.cnt[5]++;
     }
 
     switch (C.st) {
     case ST_FEED:
-        C.a1 = act ? exp(ev) * C.z : 0.0;
+        SLOTS C.a1[c] = ln.act[c] ? exp(ev[c]) * C.z[c] : 0.0;
         C.k = 0;
         C.st = C.lNP == 0 ? ST_T_START : ST_A_START;
         break;
     case ST_T_START:
     case ST_T_SSM: {
-        const double phS = exp(ev);
-        const double r = act ? log((phS * C.y * C.P) / (C.a1 * C.P) * C.Yt) : 0.0;
-        const double Se = tsum<T>(r * r);
-        C.a3 = -r;
-        C.a2 = act ? phS : 0.0;
+        double r[L];
+        SLOTS {
+            const double phS = exp(ev[c]);
+            r[c] = ln.act[c] ? log((phS * C.y[c] * C.P) / (C.a1[c] * C.P) * C.Yt) : 0.0;
+            C.a3[c] = -r[c];
+            C.a2[c] = ln.act[c] ? phS : 0.0;
+        }
+        const double Se = TSUM(r[c] * r[c]);
         if (C.st == ST_T_START) {
             C.iter = 0;
             if (Se > f.SSMsta_tol2) { C.st = ST_T_SSM; break; }
-            nr_sta_begin(f, lf, C, io, act, t);
+            nr_sta_begin(f, lf, C, io, ln);
             break;
         }
         C.cnt[0]++;
         if (Se > f.SSMsta_tol2 && ++C.iter <= f.SSMsta_maxIt) break;
-        nr_sta_begin(f, lf, C, io, act, t);
+        nr_sta_begin(f, lf, C, io, ln);
         break;
     }
     case ST_T_NR: {
-        C.a3 = act ? log((C.a1 * C.P) / ((exp(ev) * C.y * C.P) * C.Yt)) : 0.0;
-        const double e2 = tsum<T>(C.a3 * C.a3);
+        SLOTS C.a3[c] = ln.act[c] ? log((C.a1[c] * C.P) / ((exp(ev[c]) * C.y[c] * C.P) * C.Yt)) : 0.0;
+        const double e2 = TSUM(C.a3[c] * C.a3[c]);
         C.cnt[1]++;
-        if (++C.iter > f.NRsta_maxIt) { C.flag = 0; strict_decide(f, lf, C, io, t); break; }
+        if (++C.iter > f.NRsta_maxIt) { C.flag = 0; strict_decide(f, lf, C, io, ln); break; }
         if (sqrt(e2) > f.NRsta_tol) break;
         C.flag = 1;
-        strict_decide(f, lf, C, io, t);
+        strict_decide(f, lf, C, io, ln);
         break;
     }
     case ST_A_START:
     case ST_A_ITER: {
-        if (!eval) { relaxed_trial_end(f, lf, C, io, t); break; }
-        const double di = (C.a1 * C.P) / ((exp(ev) * C.y * C.P) * C.Yt);
-        C.a4 = act ? C.a4 * di : 0.0;
-        const double lk = act ? log(C.a4) : 0.0;
-        const double Se = tsum<T>(act ? (di - 1) * (di - 1) : 0.0);
-        const double Sk = tsum<T>(lk * lk);
-        C.a3 = C.y;
-        C.a2 = ev;
+        if (!eval) { relaxed_trial_end(f, lf, C, io, ln); break; }
+        double di[L], lk[L];
+        SLOTS {
+            di[c] = (C.a1[c] * C.P) / ((exp(ev[c]) * C.y[c] * C.P) * C.Yt);
+            C.a4[c] = ln.act[c] ? C.a4[c] * di[c] : 0.0;
+            lk[c] = ln.act[c] ? log(C.a4[c]) : 0.0;
+        }
+        const double Se = TSUM((di[c] - 1) * (di[c] - 1));
+        const double Sk = TSUM(lk[c] * lk[c]);
+        SLOTS { C.a3[c] = C.y[c]; C.a2[c] = ev[c]; }
         C.Sk = Sk;
         C.cnt[0]++;
         C.iter++;
-        if (Se < f.SSMsta_tol2 || Sk < KTOL) { C.flag = 1; relaxed_trial_end(f, lf, C, io, t); break; }
-        if (C.iter > f.SSMsta_maxIt) { C.flag = 0; relaxed_trial_end(f, lf, C, io, t); break; }
+        if (Se < f.SSMsta_tol2 || Sk < KTOL) { C.flag = 1; relaxed_trial_end(f, lf, C, io, ln); break; }
+        if (C.iter > f.SSMsta_maxIt) { C.flag = 0; relaxed_trial_end(f, lf, C, io, ln); break; }
         C.st = ST_A_ITER;
         break;
     }
     case ST_S_SSM_A:
-        C.a5 = ev;
+        SLOTS C.a5[c] = ev[c];
         C.st = ST_S_SSM_B;
         break;
     case ST_S_SSM_B: {
         C.cnt[2]++;
         C.iter++;
-        const double phi0 = exp(C.a5), phi1 = exp(ev);
-        const double q = act ? (phi1 * C.a3 * C.P) / (phi0 * C.a2 * C.P) - 1 : 0.0;
-        const double Se = tsum<T>(q * q);
+        double q[L], Kn[L];
+        SLOTS {
+            const double phi0 = exp(C.a5[c]), phi1 = exp(ev[c]);
+            q[c] = ln.act[c] ? (phi1 * C.a3[c] * C.P) / (phi0 * C.a2[c] * C.P) - 1 : 0.0;
+            Kn[c] = ln.act[c] ? phi1 / phi0 : 0.0;
+        }
+        const double Se = TSUM(q[c] * q[c]);
         bool last;
         if (C.iter > C.maxIt) { C.ssm_conv = 0; last = true; }
         else last = !(Se > C.tol2);
-        C.a4 = act ? phi1 / phi0 : 0.0;
-        if (last) split_nr_begin(f, lf, C, io, ev, act, t);
+        SLOTS C.a4[c] = Kn[c];
+        if (last) split_nr_begin(f, lf, C, io, ev, ln);
         else C.st = ST_S_SSM_A;
         break;
     }
     case ST_S_NR: {
-        const double lnphi0 = ev;
-        C.y = C.a3;
-        mix<NC, T>(lf, C, act, C.y, C.s, C.A, C.B, C.Z);
-        const double lnphi1 = act ? lnphi_of(C.s, C.Bi, C.A, C.B, C.Z) : 0.0;
+        double lnphi1[L];
+        SLOTS C.y[c] = C.a3[c];
+        mix<NC, T>(lf, ln, C, C.y, C.s, C.A, C.B, C.Z);
+        SLOTS lnphi1[c] = ln.act[c] ? lnphi_of(C.s[c], C.Bi[c], C.A, C.B, C.Z) : 0.0;
         C.cnt[5]++;
-        C.a5 = act ? log((exp(lnphi1) * C.a3 * C.P) / (exp(lnphi0) * C.a2 * C.P)) : 0.0;
-        const double eNR = sqrt(tsum<T>(C.a5 * C.a5));
+        SLOTS C.a5[c] = ln.act[c] ? log((exp(lnphi1[c]) * C.a3[c] * C.P) / (exp(ev[c]) * C.a2[c] * C.P)) : 0.0;
+        const double eNR = sqrt(TSUM(C.a5[c] * C.a5[c]));
         C.cnt[3]++;
         C.iter++;
         bool fin = false;
@@ -675,7 +813,7 @@ DEV void step(const DevFluid& f, const LaneFluid<NC>& lf, const PeIO& io, Cell<N
         else if (!(eNR > f.NRsp_tol)) fin = true;
         if (fin) {
             if (eNR < f.NRsp_tol) C.nr_conv = 1;
-            after_split_nr(f, lf, C, io, act, t);
+            after_split_nr(f, lf, C, io, ln);
         }
         break;
     }
@@ -700,20 +838,27 @@ DEV const DevFluid& fluid_of(const DevFluid* fp, const Cell<NC, T>& C)
     else return *opaque(fp);
 }
 
+template <int NC, int T>
+DEV Lane<NC, T> make_lane()
+{
+    Lane<NC, T> ln;
+    ln.t = __lane_id() & (T - 1);
+    UNROLL for (int c = 0; c < Lane<NC, T>::L; c++) ln.act[c] = c * T + ln.t < NC;
+    return ln;
+}
+
 template <int NC, int T, bool MULTI>
 __global__ void __launch_bounds__(64) pe_team_plain(const DevFluid* __restrict__ fp, PeIO io)
 {
-    const int lane = __lane_id();
-    const int t = lane & (T - 1);
+    const Lane<NC, T> ln = make_lane<NC, T>();
     const int cell = (blockIdx.x * 64 + threadIdx.x) / T;
     if (cell >= io.nb) return;                   // whole teams leave together
-    const bool act = t < NC;
-    LaneFluid<NC> lf;
+    LaneFluid<NC, T> lf;
     Cell<NC, T> C;
     C.model = MULTI ? io.mid[cell] : 0;
-    begin<NC, T>(fp[C.model], lf, io, cell, C, act, t);
-    while (C.st != ST_DONE) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, act, t);
-    end<NC, T>(io, C, act, t);
+    begin<NC, T>(fp[C.model], lf, io, cell, C, ln);
+    while (C.st != ST_DONE) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, ln);
+    end<NC, T>(io, C, ln);
 }
 
 // Persistent teams over the global work queue (counter io.queue, zeroed
@@ -723,19 +868,18 @@ __global__ void __launch_bounds__(64) pe_team_plain(const DevFluid* __restrict__
 template <int NC, int T, bool MULTI>
 __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict__ fp, PeIO io)
 {
+    const Lane<NC, T> ln = make_lane<NC, T>();
     const int lane = __lane_id();
-    const int t = lane & (T - 1);
     const int lead = lane & ~(T - 1);
-    const bool act = t < NC;
-    LaneFluid<NC> lf;
+    LaneFluid<NC, T> lf;
     Cell<NC, T> C;
     C.st = ST_DONE;
     C.model = 0;
     bool have = false, drained = false;
     while (true) {
-        if (C.st == ST_DONE && have) { end<NC, T>(io, C, act, t); have = false; }
+        if (C.st == ST_DONE && have) { end<NC, T>(io, C, ln); have = false; }
         const bool need = !have && !drained;
-        const unsigned long long m = __ballot(need && t == 0);
+        const unsigned long long m = __ballot(need && ln.t == 0);
         if (m) {
             const int first = __ffsll((long long)m) - 1;
             unsigned base = 0;
@@ -745,14 +889,14 @@ __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lead) - 1ull));
                 if (idx < (unsigned)io.nb) {
                     C.model = MULTI ? io.mid[idx] : 0;
-                    begin<NC, T>(fp[C.model], lf, io, (int)idx, C, act, t);
+                    begin<NC, T>(fp[C.model], lf, io, (int)idx, C, ln);
                     have = true;
                 }
                 else drained = true;
             }
         }
         if (!__any(have)) break;
-        if (have && C.st != ST_DONE) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, act, t);
+        if (have && C.st != ST_DONE) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, ln);
     }
 }
 
@@ -764,8 +908,8 @@ hipError_t launch(const DevFluid* f, const PeIO& io, bool batched, int grid_bloc
     long need = (lanes + 63) / 64;
     const bool multi = io.mid != nullptr;
     if (!batched) {
-        if (multi) hipLaunchKernelGGL((pe_team_plain<NC, T, true>), dim3((unsigned)need), dim3(64), 0, st, f, io);
-        else hipLaunchKernelGGL((pe_team_plain<NC, T, false>), dim3((unsigned)need), dim3(64), 0, st, f, io);
+        if (multi) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((pe_team_plain<NC, T, false>), dim3((unsigned)need), dim3(64), 0, st, f, io);
     } else {
         const long nblk = grid_blocks < need ? grid_blocks : need;
         if (multi) hipLaunchKernelGGL((pe_team_batched<NC, T, true>), dim3((unsigned)nblk), dim3(64), 0, st, f, io);
@@ -784,35 +928,49 @@ int blocks_per_cu()
 
 }  // namespace team
 
-// T = 8 for Nc <= 8, 16 above (teams tile a 64-lane wavefront exactly)
+// Team sizes built per Nc: 4 for every Nc (1-3 components per lane), 8 from
+// Nc = 5 (1-2 per lane), 16 from Nc = 9 (one per lane).
+bool pe_team_size_ok(int NC, int T)
+{
+    if (NC < 2 || NC > 12) return false;
+    return T == 4 || (T == 8 && NC >= 5) || (T == 16 && NC >= 9);
+}
+
 hipError_t run_pe_team(int NC, int T, const DevFluid* f, const PeIO& io, bool batched, int grid_blocks,
                        hipStream_t st)
 {
-#define OCPX_T8(N) case N: return T == 8 ? team::launch<N, 8>(f, io, batched, grid_blocks, st) \
-                                         : team::launch<N, 16>(f, io, batched, grid_blocks, st);
-#define OCPX_T16(N) case N: return team::launch<N, 16>(f, io, batched, grid_blocks, st);
-    if (T != 8 && T != 16) return hipErrorInvalidValue;
-    if (T == 8 && NC > 8) return hipErrorInvalidValue;
+    if (!pe_team_size_ok(NC, T)) return hipErrorInvalidValue;
+#define OCPX_T4(N) case N: return team::launch<N, 4>(f, io, batched, grid_blocks, st);
+#define OCPX_T48(N) case N: return T == 4 ? team::launch<N, 4>(f, io, batched, grid_blocks, st) \
+                                          : team::launch<N, 8>(f, io, batched, grid_blocks, st);
+#define OCPX_T4816(N) case N: return T == 4 ? team::launch<N, 4>(f, io, batched, grid_blocks, st) \
+                                 : (T == 8 ? team::launch<N, 8>(f, io, batched, grid_blocks, st) \
+                                           : team::launch<N, 16>(f, io, batched, grid_blocks, st));
     switch (NC) {
-        OCPX_T8(2) OCPX_T8(3) OCPX_T8(4) OCPX_T8(5) OCPX_T8(6) OCPX_T8(7) OCPX_T8(8)
-        OCPX_T16(9) OCPX_T16(10) OCPX_T16(11) OCPX_T16(12)
+        OCPX_T4(2) OCPX_T4(3) OCPX_T4(4) OCPX_T48(5) OCPX_T48(6) OCPX_T48(7) OCPX_T48(8)
+        OCPX_T4816(9) OCPX_T4816(10) OCPX_T4816(11) OCPX_T4816(12)
     default: return hipErrorInvalidValue;
     }
-#undef OCPX_T8
-#undef OCPX_T16
+#undef OCPX_T4
+#undef OCPX_T48
+#undef OCPX_T4816
 }
 
 int pe_team_blocks_per_cu(int NC, int T)
 {
+    if (!pe_team_size_ok(NC, T)) return 0;
+#define OCPX_B4(N) case N: return team::blocks_per_cu<N, 4>();
+#define OCPX_B48(N) case N: return T == 4 ? team::blocks_per_cu<N, 4>() : team::blocks_per_cu<N, 8>();
+#define OCPX_B4816(N) case N: return T == 4 ? team::blocks_per_cu<N, 4>() \
+                                 : (T == 8 ? team::blocks_per_cu<N, 8>() : team::blocks_per_cu<N, 16>());
     switch (NC) {
-#define OCPX_B8(N) case N: return T == 8 ? team::blocks_per_cu<N, 8>() : team::blocks_per_cu<N, 16>();
-#define OCPX_B16(N) case N: return T == 16 ? team::blocks_per_cu<N, 16>() : 0;
-        OCPX_B8(2) OCPX_B8(3) OCPX_B8(4) OCPX_B8(5) OCPX_B8(6) OCPX_B8(7) OCPX_B8(8)
-        OCPX_B16(9) OCPX_B16(10) OCPX_B16(11) OCPX_B16(12)
-#undef OCPX_B8
-#undef OCPX_B16
+        OCPX_B4(2) OCPX_B4(3) OCPX_B4(4) OCPX_B48(5) OCPX_B48(6) OCPX_B48(7) OCPX_B48(8)
+        OCPX_B4816(9) OCPX_B4816(10) OCPX_B4816(11) OCPX_B4816(12)
     default: return 0;
     }
+#undef OCPX_B4
+#undef OCPX_B48
+#undef OCPX_B4816
 }
 
 }  // namespace ocpx

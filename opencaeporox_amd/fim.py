@@ -147,9 +147,9 @@ class IsoTFIM:
     def autotune_flash(self):
         """Times the batched phase-equilibrium kernel per mapping on the
         current state; keeps the fastest.  Returns {lanes_per_cell: ms}."""
-        ms = np.zeros(3)
+        ms = np.zeros(4)
         self._chk(self.lib.ocpx_autotune_flash(self.h, ms.ctypes.data), "autotune")
-        return {k: v for k, v in zip((1, 8, 16), ms) if v > 0}
+        return {k: v for k, v in zip((1, 4, 8, 16), ms) if v > 0}
 
     def CalFlash(self, mode=PLAIN):
         self._chk(self.lib.ocpx_flash_der(self.h, mode), "flash")

@@ -25,16 +25,17 @@ def _models():
     return out
 
 
-def test_merged_context_equals_separate_contexts():
+@pytest.mark.parametrize("lanes", [4, 8])
+def test_merged_context_equals_separate_contexts(lanes):
     wls = _models()
     merged = IsoTFIM.from_workloads(wls)
-    merged.set_flash_mapping(8)
+    merged.set_flash_mapping(lanes)
     merged.newton_pass(1.0, BATCHED)
     rp, col, val, rhs = merged.bsr()
     off = merged.offsets
     for m, wl in enumerate(wls):
         one = IsoTFIM.from_workload(wl)
-        one.set_flash_mapping(8)
+        one.set_flash_mapping(lanes)
         one.newton_pass(1.0, BATCHED)
         a, b = off[m], off[m + 1]
         for f in ("PHASEEXIST", "S", "XIJ", "DSEC_DPRI", "KR", "PORO"):

@@ -65,7 +65,7 @@ def _compare_flash(dev, st, Nc, tag, others, max_mismatch=0.02):
     return ~mism
 
 
-@pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 8), (BATCHED, 16)])
+@pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 4), (BATCHED, 8)])
 def test_flash_config1(oracle, oracle_fma, mode, lanes):
     wl = W.config1()
     dev = IsoTFIM.from_workload(wl)
@@ -82,8 +82,10 @@ def test_flash_config1(oracle, oracle_fma, mode, lanes):
     assert ok, f"minEig worst {worst}"
 
 
+# team sizes built per Nc: 4 always, 8 from Nc = 5, 16 from Nc = 9
 PE_CASES = [(Nc, m, lanes) for Nc in (3, 4, 5, 9, 12)
-            for m, lanes in ((PLAIN, 1), (BATCHED, 1), (BATCHED, 8 if Nc <= 8 else 16))]
+            for m, lanes in ((PLAIN, 1), (BATCHED, 1), (BATCHED, 4), (BATCHED, 8), (BATCHED, 16))
+            if lanes < 8 or (lanes == 8 and Nc >= 5) or (lanes == 16 and Nc >= 9)]
 
 
 @pytest.mark.parametrize("Nc,mode,lanes", PE_CASES)
@@ -213,7 +215,7 @@ def _clean_grid(seed=31, Nc=3, nx=4, ny=3, nz=2):
     return W.Workload("clean grid", fl, W.make_sat(), rock, grid, stt)
 
 
-@pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 8)])
+@pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 4)])
 def test_newton_passes_end_to_end(oracle, oracle_fma, mode, lanes):
     """Device flash + assembly against the oracle's on a grid whose cells are
     all well-conditioned, then GetSolution and a second pass from the
@@ -281,7 +283,7 @@ def test_autotuner_picks_a_valid_mapping():
     wl, _ = W.flash_cells(20000, 6, seed=3)
     dev = IsoTFIM.from_workload(wl)
     ms = dev.autotune_flash()
-    assert set(ms) == {1, 8, 16} and all(v > 0 for v in ms.values())
+    assert set(ms) == {1, 4, 8} and all(v > 0 for v in ms.values())
     assert dev.flash_mapping() == min(ms, key=ms.get)
     dev.set_flash_mapping(0)
     assert dev.flash_mapping() == min(ms, key=ms.get)
