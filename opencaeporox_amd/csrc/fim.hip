@@ -7,22 +7,25 @@
 // 24-70) runs in two passes, both atomic-free:
 //
 //  1. krpc_aux (one cell per b = nc+1 lanes, lane r = primary variable r):
-//     kr/Pc and rock as in the reference, then, per phase j, the total
-//     derivatives with respect to the primary variables of the three
-//     quantities a connection needs: the mobility lambda_j = xi_j kr_j / mu_j,
-//     the phase pressure P_j = P + Pc_j and the mass density rho_j, chained
+//     kr/Pc and rock as in the reference, then the cell's connection record:
+//     per phase j the scalars a connection reads (rho, Pj, xi, kr, mu,
+//     existence, depth), x_ij, and the total derivatives with respect to
+//     the primary variables of the mobility lambda_j = xi_j kr_j / mu_j, the
+//     phase pressure P_j = P + Pc_j and the mass density rho_j, chained
 //     through dXs/dXp once per cell.
-//  2. res_asm (one block row per b lanes, lane i = equation i): for each
-//     connection of the row, in OCPMatrix push order, the flux of component i
-//     F_i = T lambda_U x_ijU dPhi_j and its row of dF/dX on either side,
+//  2. res_asm (one block row per b lanes, lane r = block column r): for each
+//     connection of the row, in OCPMatrix push order, the flux
+//     F_i = T lambda_U x_ijU dPhi_j and column r of dF/dX on either side,
 //         dF_i/dX_s = T dPhi_j [s == U] (x_ij dlambda_U + lambda_U dx_ij,U)
 //                   + T lambda_U x_ij dPhi_j/dX_s,
 //     with dPhi_j/dX_b = dP_jb - g dz w_b drho_jb, dPhi_j/dX_e = -dP_je -
 //     g dz w_e drho_je.  That is the reference's dt (dF/dXp + dF/dXs dXs/dXp)
 //     with the product taken once per cell instead of once per connection.
-//     Each lane writes one contiguous row of each block of its row.  Both
-//     rows of a connection compute its flux; that doubled arithmetic is
+//     A lane reads one element of each derivative vector (the B lanes of a
+//     row read B contiguous doubles) and writes column r of each block.
+//     Both rows of a connection compute its flux; that doubled arithmetic is
 //     cheaper than a connection-ordered scatter.
+//  res_only is the residual alone (CalRes without a Jacobian).
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 
@@ -84,13 +87,13 @@ OCPX_HD void krpc_cell(const DevSat& s, double Sg, double Sw, double* kr, double
     }
 }
 
-// Connection aux record of a cell, per phase j at aux + (n*3 + j)*AUXJ:
-// [dlambda[b], dPj[b], drho[b], lambda], AUXJ = 3b + 1.
+// Also writes the cell's connection record (aux_len layout) for the assembly.
 template <int NCB>
 __global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, const DevRock* __restrict__ rp,
                                                CellIO c)
 {
-    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, AUXJ = 3 * B + 1;
+    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, RL = aux_len(B, nc);
+    constexpr int SV = 24, SX = 24 + 9 * B;
     const int lane = threadIdx.x, g = lane / B, r = lane % B;
     const int64_t n = (int64_t)blockIdx.x * G + g;
     if (g >= G || n >= c.nb) return;
@@ -118,7 +121,7 @@ __global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, co
     const double* X = c.dXsdXp + (size_t)n * REC;     // row k, column r: X[k * B + r]
     double dS[np];
     UNROLL for (int k = 0; k < np; k++) dS[k] = X[k * B + r];
-    double* A = c.aux + (size_t)n * np * AUXJ;
+    double* A = c.aux + (size_t)n * RL;
     const double one = r == 0 ? 1.0 : 0.0;
     UNROLL for (int j = 0; j < np; j++) {
         const bool ex = c.phaseExist[j * nb + n] != 0;
@@ -136,16 +139,27 @@ __global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, co
         }
         double dkr = 0, dPj = one;
         UNROLL for (int k = 0; k < np; k++) { dkr += dk[j * 3 + k] * dS[k]; dPj += dp[j * 3 + k] * dS[k]; }
+        const double xi = c.xi[j * nb + n], mu = c.mu[j * nb + n];
         double lam = 0, dlam = 0;
         if (ex) {
-            const double xi = c.xi[j * nb + n], mu = c.mu[j * nb + n];
             lam = xi * kr[j] / mu;
             dlam = kr[j] / mu * dxi + xi / mu * dkr - xi * kr[j] / (mu * mu) * dmu;
         }
-        A[j * AUXJ + r] = dlam;
-        A[j * AUXJ + B + r] = dPj;
-        A[j * AUXJ + 2 * B + r] = drho;
-        if (r == 0) A[j * AUXJ + 3 * B] = lam;
+        A[SV + 3 * B * j + r] = dlam;
+        A[SV + 3 * B * j + B + r] = dPj;
+        A[SV + 3 * B * j + 2 * B + r] = drho;
+        if (r < nc) A[SX + j * nc + r] = c.xij[(size_t)(j * nc + r) * nb + n];
+        if (r == 0) {
+            double* Sj = A + 8 * j;
+            Sj[0] = c.rho[j * nb + n];
+            Sj[1] = P + pc[j];
+            Sj[2] = xi;
+            Sj[3] = kr[j];
+            Sj[4] = mu;
+            Sj[5] = ex ? 1.0 : 0.0;
+            Sj[6] = lam;
+            Sj[7] = c.depth[n];
+        }
     }
 }
 
@@ -182,95 +196,15 @@ OCPX_HD bool upwind(const CellIO& c, size_t nb, int64_t b, int64_t e, int j, Upw
     return exu;
 }
 
-template <int NCB>
-__global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, double dt, int do_res,
-                                              int do_jac, double* __restrict__ resAbs,
-                                              double* __restrict__ bmax, int64_t* __restrict__ barg)
+// Block reduction of max |R/Vp| and max |R_i/Nt| (first cell on ties) into
+// bmax/barg[2 * block + {0, 1}].
+__device__ __forceinline__ void block_norms(double absV, double absN, int64_t idx, double* bmax, int64_t* barg)
 {
-    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, AUXJ = 3 * B + 1;
-    const int lane = threadIdx.x;
-    const int g = lane / B, i = lane % B;              // i: equation (block row)
-    const size_t nb = c.nb;
-    const int64_t blk = xcd_chunk(blockIdx.x, gridDim.x);
-    const int64_t p = blk * G + g;                     // position in the row order
-    const bool valid = (g < G) && (p < c.nb);
-    const int64_t n = valid ? (bsr.order ? (int64_t)bsr.order[p] : p) : 0;
     __shared__ double sV[64], sN[64];
     __shared__ int64_t iV[64], iN[64];
-
-    double res = 0, absV = -1, absN = -1;
-    if (valid) {
-        // accumulation (BulkAccumuTerm01::CalResFIM / CaldFdXpFIM)
-        double diag[B];
-        if (do_jac) {
-            UNROLL for (int k = 0; k < B; k++) diag[k] = (i > 0 && k == i) ? 1.0 : 0.0;
-            if (i == 0) {
-                diag[0] = c.v[n] * c.poroP[n] - c.vfP[n];
-                UNROLL for (int k = 1; k < B; k++) diag[k] = -c.vfi[(size_t)(k - 1) * nb + n];
-            }
-        }
-        if (do_res) res = i == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(i - 1) * nb + n] - c.lNi[(size_t)(i - 1) * nb + n];
-
-        const int64_t s0 = bsr.rowptr[n], s1 = bsr.rowptr[n + 1];
-        for (int64_t s = s0 + 1; s < s1; s++) {
-            const int cc = bsr.nbr_conn[s];
-            const int64_t bI = cn.cb[cc], eI = cn.ce[cc];
-            const bool ownB = (bI == n);
-            const double T = cn.trans[cc];
-            const double dGamma = GRAVF * (c.depth[bI] - c.depth[eI]);
-            double rb[B], re[B];                        // row i of dF/dX_b and dF/dX_e
-            UNROLL for (int k = 0; k < B; k++) { rb[k] = 0; re[k] = 0; }
-            double flux = 0;
-            UNROLL for (int j = 0; j < np; j++) {
-                Upw u;
-                if (!upwind(c, nb, bI, eI, j, u)) continue;
-                if (i == 0) continue;                   // the volume row carries no flux
-                const int64_t U = u.upIsB ? bI : eI;
-                const double xi = c.xi[j * nb + U], kr = c.kr[j * nb + U], mu = c.mu[j * nb + U];
-                const double tJ = T * kr / mu;
-                const double xij = c.xij[(size_t)(j * nc + i - 1) * nb + U];
-                flux += tJ * u.dP * xi * xij;
-                if (!do_jac) continue;
-                const double tIJ = xij * xi * tJ;
-                const double wU = u.downExists ? 0.5 : 1.0, wD = u.downExists ? 0.5 : 0.0;
-                const double wb = u.upIsB ? wU : wD, we = u.upIsB ? wD : wU;
-                const double* Ab = c.aux + ((size_t)bI * np + j) * AUXJ;
-                const double* Ae = c.aux + ((size_t)eI * np + j) * AUXJ;
-                const double* AU = u.upIsB ? Ab : Ae;
-                const double* XU = c.dXsdXp + (size_t)U * REC + (np + j * nc + i - 1) * B;
-                const double lam = AU[3 * B], Tdp = T * u.dP;
-                UNROLL for (int r = 0; r < B; r++) {
-                    const double up = Tdp * (xij * AU[r] + lam * XU[r]);
-                    rb[r] += tIJ * (Ab[B + r] - dGamma * (wb * Ab[2 * B + r])) + (u.upIsB ? up : 0.0);
-                    re[r] += tIJ * (-Ae[B + r] - dGamma * (we * Ae[2 * B + r])) + (u.upIsB ? 0.0 : up);
-                }
-            }
-            if (do_res && i > 0) res += ownB ? dt * flux : -dt * flux;
-            if (do_jac) {
-                double* off = bsr.val + ((size_t)s * B + i) * B;
-                if (ownB) {                       // AddDiag(b, B); NewOffDiag(b, e, E)
-                    UNROLL for (int r = 0; r < B; r++) { diag[r] += dt * rb[r]; off[r] = dt * re[r]; }
-                } else {                          // NewOffDiag(e, b, -B); AddDiag(e, -E)
-                    UNROLL for (int r = 0; r < B; r++) { off[r] = -(dt * rb[r]); diag[r] += -(dt * re[r]); }
-                }
-            }
-        }
-        if (do_jac) {
-            double* dg = bsr.val + ((size_t)s0 * B + i) * B;
-            UNROLL for (int r = 0; r < B; r++) dg[r] = diag[r];
-        }
-        if (do_res) {
-            const double out = -res;             // Dscalar(-1) :1170
-            resAbs[(size_t)n * B + i] = out;
-            if (bsr.rhs) bsr.rhs[(size_t)n * B + i] = out;   // CopyRhs
-            absV = fabs(res / c.rockVp[n]);
-            if (i > 0) absN = fabs(res / c.Nt[n]);
-        }
-    }
-    if (!do_res) return;
-    // block reduction of max |R/Vp| and max |R_i/Nt| (first cell on ties)
-    sV[lane] = absV; iV[lane] = valid ? n : INT64_MAX;
-    sN[lane] = absN; iN[lane] = valid ? n : INT64_MAX;
+    const int lane = threadIdx.x;
+    sV[lane] = absV; iV[lane] = idx;
+    sN[lane] = absN; iN[lane] = idx;
     __syncthreads();
     for (int w = 32; w > 0; w >>= 1) {
         if (lane < w) {
@@ -284,6 +218,150 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
         bmax[2 * blockIdx.x] = sV[0]; barg[2 * blockIdx.x] = iV[0];
         bmax[2 * blockIdx.x + 1] = sN[0]; barg[2 * blockIdx.x + 1] = iN[0];
     }
+}
+
+// Residual only (IsoT_FIM::CalRes), one block row per b lanes, lane i =
+// equation i; reads the cell state arrays directly.
+template <int NCB>
+__global__ void __launch_bounds__(64) res_only(CellIO c, ConnIO cn, BsrIO bsr, double dt,
+                                               double* __restrict__ resAbs, double* __restrict__ bmax,
+                                               int64_t* __restrict__ barg)
+{
+    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B;
+    const int lane = threadIdx.x;
+    const int g = lane / B, i = lane % B;
+    const size_t nb = c.nb;
+    const int64_t blk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int64_t p = blk * G + g;
+    const bool valid = (g < G) && (p < c.nb);
+    const int64_t n = valid ? (bsr.order ? (int64_t)bsr.order[p] : p) : 0;
+    double absV = -1, absN = -1;
+    if (valid) {
+        double res = i == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(i - 1) * nb + n] - c.lNi[(size_t)(i - 1) * nb + n];
+        const int64_t s0 = bsr.rowptr[n], s1 = bsr.rowptr[n + 1];
+        for (int64_t s = s0 + 1; s < s1 && i > 0; s++) {
+            const int cc = bsr.nbr_conn[s];
+            const int64_t bI = cn.cb[cc], eI = cn.ce[cc];
+            const double T = cn.trans[cc];
+            double flux = 0;
+            UNROLL for (int j = 0; j < np; j++) {
+                Upw u;
+                if (!upwind(c, nb, bI, eI, j, u)) continue;
+                const int64_t U = u.upIsB ? bI : eI;
+                const double tJ = T * c.kr[j * nb + U] / c.mu[j * nb + U];
+                flux += tJ * u.dP * c.xi[j * nb + U] * c.xij[(size_t)(j * nc + i - 1) * nb + U];
+            }
+            res += bI == n ? dt * flux : -dt * flux;
+        }
+        const double out = -res;             // Dscalar(-1) :1170
+        resAbs[(size_t)n * B + i] = out;
+        if (bsr.rhs) bsr.rhs[(size_t)n * B + i] = out;   // CopyRhs
+        absV = fabs(res / c.rockVp[n]);
+        if (i > 0) absN = fabs(res / c.Nt[n]);
+    }
+    block_norms(absV, absN, valid ? n : INT64_MAX, bmax, barg);
+}
+
+// Residual + Jacobian (CalRes + AssembleMatBulks), one block row per b
+// lanes, lane r = block column r (and equation r of the residual).  Every
+// load of a connection's vectors is one element per lane, 64 contiguous
+// bytes per row; the cell scalars come from the connection record.
+template <int NCB>
+__global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, double dt,
+                                              double* __restrict__ resAbs, double* __restrict__ bmax,
+                                              int64_t* __restrict__ barg)
+{
+    constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, RL = aux_len(B, nc);
+    constexpr int SV = 24, SX = 24 + 9 * B;
+    const int lane = threadIdx.x;
+    const int g = lane / B, r = lane % B;
+    const size_t nb = c.nb;
+    const int64_t blk = xcd_chunk(blockIdx.x, gridDim.x);
+    const int64_t p = blk * G + g;                     // position in the row order
+    const bool valid = (g < G) && (p < c.nb);
+    const int64_t n = valid ? (bsr.order ? (int64_t)bsr.order[p] : p) : 0;
+    double absV = -1, absN = -1;
+    if (valid) {
+        // accumulation (BulkAccumuTerm01::CalResFIM / CaldFdXpFIM), column r
+        double diag[B];
+        diag[0] = r == 0 ? c.v[n] * c.poroP[n] - c.vfP[n] : -c.vfi[(size_t)(r - 1) * nb + n];
+        UNROLL for (int i = 1; i < B; i++) diag[i] = (r == i) ? 1.0 : 0.0;
+        double res = r == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(r - 1) * nb + n] - c.lNi[(size_t)(r - 1) * nb + n];
+
+        const int64_t s0 = bsr.rowptr[n], s1 = bsr.rowptr[n + 1];
+        for (int64_t s = s0 + 1; s < s1; s++) {
+            const int cc = bsr.nbr_conn[s];
+            const int64_t bI = cn.cb[cc], eI = cn.ce[cc];
+            const bool ownB = (bI == n);
+            const double T = cn.trans[cc];
+            const double* Ab = c.aux + (size_t)bI * RL;
+            const double* Ae = c.aux + (size_t)eI * RL;
+            const double zb = Ab[7], ze = Ae[7];
+            const double dGamma = GRAVF * (zb - ze);
+            double rb[nc], re[nc];                     // column r of dF/dX_b and dF/dX_e, rows 1..nc
+            UNROLL for (int i = 0; i < nc; i++) { rb[i] = 0; re[i] = 0; }
+            double flux = 0;
+            UNROLL for (int j = 0; j < np; j++) {
+                // upwinding (OCPConvection01::CalFlux :37-78)
+                const double* Sb = Ab + 8 * j;
+                const double* Se = Ae + 8 * j;
+                const bool exb = Sb[5] != 0, exe = Se[5] != 0;
+                if (!exb && !exe) continue;
+                double rho;
+                if (exb && exe) rho = (Sb[0] + Se[0]) / 2;
+                else if (exb) rho = Sb[0];
+                else rho = Se[0];
+                const double dP = (Sb[1] - GRAVF * rho * zb) - (Se[1] - GRAVF * rho * ze);
+                const bool upIsB = !(dP < 0);
+                if (!(upIsB ? exb : exe)) continue;
+                const bool downExists = upIsB ? exe : exb;
+                const double* AU = upIsB ? Ab : Ae;
+                const int64_t U = upIsB ? bI : eI;
+                const double xi = AU[8 * j + 2], kr = AU[8 * j + 3], mu = AU[8 * j + 4], lam = AU[8 * j + 6];
+                const double tJ = T * kr / mu;
+                double x[nc];
+                UNROLL for (int i = 0; i < nc; i++) x[i] = AU[SX + j * nc + i];
+                double xr = 0;                          // component of equation r
+                UNROLL for (int i = 0; i < nc; i++) xr = (r == i + 1) ? x[i] : xr;
+                flux += tJ * dP * xi * xr;
+                const double wU = downExists ? 0.5 : 1.0, wD = downExists ? 0.5 : 0.0;
+                const double wb = upIsB ? wU : wD, we = upIsB ? wD : wU;
+                const double* Vb = Ab + SV + 3 * B * j;
+                const double* Ve = Ae + SV + 3 * B * j;
+                const double vb = Vb[B + r] - dGamma * (wb * Vb[2 * B + r]);
+                const double ve = -Ve[B + r] - dGamma * (we * Ve[2 * B + r]);
+                const double dl = (upIsB ? Vb : Ve)[r];
+                const double Tdp = T * dP;
+                const double* XU = c.dXsdXp + (size_t)U * REC + (size_t)(np + j * nc) * B + r;
+                UNROLL for (int i = 0; i < nc; i++) {
+                    const double tIJ = x[i] * xi * tJ;
+                    const double up = Tdp * (x[i] * dl + lam * XU[i * B]);
+                    rb[i] += tIJ * vb + (upIsB ? up : 0.0);
+                    re[i] += tIJ * ve + (upIsB ? 0.0 : up);
+                }
+            }
+            if (r > 0) res += ownB ? dt * flux : -dt * flux;
+            double* off = bsr.val + (size_t)s * B * B + r;        // column r of the slot's block
+            off[0] = ownB ? dt * 0.0 : -(dt * 0.0);               // the volume row carries no flux
+            UNROLL for (int i = 0; i < nc; i++) {
+                if (ownB) {                       // AddDiag(b, B); NewOffDiag(b, e, E)
+                    diag[i + 1] += dt * rb[i];
+                    off[(i + 1) * B] = dt * re[i];
+                } else {                          // NewOffDiag(e, b, -B); AddDiag(e, -E)
+                    off[(i + 1) * B] = -(dt * rb[i]);
+                    diag[i + 1] += -(dt * re[i]);
+                }
+            }
+        }
+        double* dg = bsr.val + (size_t)s0 * B * B + r;
+        UNROLL for (int i = 0; i < B; i++) dg[i * B] = diag[i];
+        const double out = -res;             // Dscalar(-1) :1170
+        resAbs[(size_t)n * B + r] = out;
+        if (bsr.rhs) bsr.rhs[(size_t)n * B + r] = out;   // CopyRhs
+        absV = fabs(res / c.rockVp[n]);
+        if (r > 0) absN = fabs(res / c.Nt[n]);
+    }
+    block_norms(absV, absN, valid ? n : INT64_MAX, bmax, barg);
 }
 
 __global__ void __launch_bounds__(256) norms_final(int64_t nblk, const double* __restrict__ bmax,
@@ -425,13 +503,13 @@ hipError_t run_krpc_rock(const DevSat* s, const DevRock* r, const CellIO& c, hip
 }
 
 template <int NCB>
-static hipError_t launch_res_asm(const CellIO& c, const ConnIO& cn, const BsrIO& bsr, double dt, bool dr,
-                                 bool dj, double* res, double* bmax, int64_t* barg, hipStream_t st)
+static hipError_t launch_res_asm(const CellIO& c, const ConnIO& cn, const BsrIO& bsr, double dt, bool dj,
+                                 double* res, double* bmax, int64_t* barg, hipStream_t st)
 {
     constexpr int G = 64 / (NCB + 1);
     const unsigned nblk = (unsigned)((c.nb + G - 1) / G);
-    hipLaunchKernelGGL(res_asm<NCB>, dim3(nblk), dim3(64), 0, st, c, cn, bsr, dt, dr ? 1 : 0, dj ? 1 : 0, res,
-                       bmax, barg);
+    if (dj) hipLaunchKernelGGL(res_asm<NCB>, dim3(nblk), dim3(64), 0, st, c, cn, bsr, dt, res, bmax, barg);
+    else hipLaunchKernelGGL(res_only<NCB>, dim3(nblk), dim3(64), 0, st, c, cn, bsr, dt, res, bmax, barg);
     return hipGetLastError();
 }
 
@@ -447,7 +525,9 @@ hipError_t run_res_asm(int nc, const CellIO& c, const ConnIO& cn, const BsrIO& b
                        bool dj, double* res, double* bmax, int64_t* barg, hipStream_t st)
 {
     if (c.nb == 0) return hipSuccess;
-#define X(N) case N: return launch_res_asm<N>(c, cn, bsr, dt, dr, dj, res, bmax, barg, st);
+    if (!dr) return hipErrorInvalidValue;      // the Jacobian pass always forms the residual too
+    if (dj && !c.aux) return hipErrorInvalidValue;
+#define X(N) case N: return launch_res_asm<N>(c, cn, bsr, dt, dj, res, bmax, barg, st);
     OCPX_NCB_SWITCH(X)
 #undef X
 }

@@ -59,7 +59,7 @@ struct ocpx_ctx {
     bool ftype_override = false;
     // geometry
     double *v = nullptr, *depth = nullptr, *poroInit = nullptr, *trans = nullptr;
-    double* aux = nullptr;       // connection aux record [nb][3][3b+1]
+    double* aux = nullptr;       // connection record [nb][aux_len(b, nc)]
     int32_t *cb = nullptr, *ce = nullptr;
     // PE results
     int32_t* pe_NP = nullptr;
@@ -474,7 +474,7 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     ALLOC(c->cb, c->nconn); ALLOC(c->ce, c->nconn); ALLOC(c->trans, c->nconn);
     ALLOC(c->pe_NP, c->nb); ALLOC(c->pe_nu, c->nb); ALLOC(c->pe_x, (size_t)2 * c->NC * c->nb);
     ALLOC(c->queue, 4);
-    ALLOC(c->aux, (size_t)c->nb * 3 * (3 * (c->nc + 1) + 1));
+    ALLOC(c->aux, (size_t)c->nb * aux_len(c->nc + 1, c->nc));
     // BSR in OCPMatrix push order: diagonal first, then per connection in
     // index order, row e gets (e, b) and row b gets (b, e)
     c->nnzb = c->nb + 2 * c->nconn;

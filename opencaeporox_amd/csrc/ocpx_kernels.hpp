@@ -66,6 +66,15 @@ hipError_t run_pe(int NC, const DevFluid* f, const PeIO& io, int mode, int grid_
 hipError_t run_tail(int NC, const DevFluid* f, const TailIO& io, hipStream_t st);
 
 // FIM kernels (fim.hip)
+
+// Connection record of a cell, aux_len(b, nc) doubles at aux + n * aux_len
+// (written by krpc_aux, read by the Jacobian assembly):
+//   [8j, 8j + 8)            phase j: rho, Pj, xi, kr, mu, exists (1/0), lambda, depth
+//   [24 + 3bj, 24 + 3b(j+1)) phase j: dlambda[b], dPj[b], drho[b] over the primary variables
+//   [24 + 9b + nc j, ...)    phase j: x_ij, i < nc
+// Even length keeps every record 16-byte aligned.
+constexpr int aux_len(int b, int nc) { return (24 + 9 * b + 3 * nc + 1) & ~1; }
+
 struct CellIO {
     int64_t nb;
     int32_t nc;                 // bulk components (Nc + 1)
@@ -74,7 +83,7 @@ struct CellIO {
     const uint32_t* phaseExist;
     double *kr, *Pc, *Pj, *dKrdS, *dPcdS, *poro, *poroP, *rockVp;
     const double *v, *depth, *poroInit;
-    double* aux;                // [nb][3][3b+1] connection aux record (krpc_aux)
+    double* aux;                // [nb][aux_len] connection record (krpc_aux)
     const int32_t* mid;         // [nb] model of each cell, or null (one model)
 };
 

@@ -8,11 +8,12 @@
 // so a cell's state is a few registers per lane instead of ~7 Nc doubles in
 // one lane, and no Nc fills private memory.
 //
-//   * sums over components are butterfly reductions inside the team after a
-//     local sum over the lane's slots (every lane ends with the same bits, so
-//     control flow stays uniform across the team);
+//   * sums over components are butterfly reductions inside the team (DPP
+//     lane permutations) after a local sum over the lane's slots; every lane
+//     ends with the same bits, so control flow stays uniform across the team;
 //   * the mixture term s_i = sqrt(Ai) sum_k (1-k_ik) sqrt(Ak) x_k takes the
-//     other components' sqrt(Ak) x_k by shuffle, summed in k order;
+//     other components' sqrt(Ak) x_k by broadcast (DPP quad_perm for teams
+//     of 4, ds_swizzle for 8 and 16), summed in k order;
 //   * Newton matrices are built row by row on the lanes owning the rows; the
 //     entries above the diagonal, which the reference takes from the lower
 //     triangle (dsysv_ 'U' on row-major storage), are evaluated on the row's
@@ -29,6 +30,9 @@
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 
+#include <type_traits>
+#include <utility>
+
 namespace ocpx {
 namespace team {
 
@@ -44,29 +48,68 @@ enum : int {
 #define DEV __device__ __forceinline__
 
 // ----------------------------------------------------------- team helpers
-template <int T>
-DEV double tsum(double v)
+// Reductions are butterflies of DPP lane permutations (quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror): each stage pairs every
+// lane with one partner in the other half of its group of 2, 4, 8 or 16
+// lanes, and the combining operation is commutative, so every lane of a
+// team ends with the same bits.
+template <int CTRL>
+DEV int dpp_i(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+DEV double dpp_d(double v)
 {
-    UNROLL for (int o = T / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, T);
+    const long long u = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL>((int)(u & 0xffffffffll));
+    const int hi = dpp_i<CTRL>((int)(u >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int T, typename Op>
+DEV double tred(double v, Op op)
+{
+    static_assert(T == 4 || T == 8 || T == 16, "team size");
+    v = op(v, dpp_d<0xB1>(v));
+    v = op(v, dpp_d<0x4E>(v));
+    if constexpr (T >= 8) v = op(v, dpp_d<0x141>(v));
+    if constexpr (T >= 16) v = op(v, dpp_d<0x140>(v));
     return v;
 }
 template <int T>
-DEV double tmin(double v)
-{
-    UNROLL for (int o = T / 2; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, T));
-    return v;
-}
+DEV double tsum(double v) { return tred<T>(v, [](double a, double b) { return a + b; }); }
 template <int T>
-DEV double tmax(double v)
-{
-    UNROLL for (int o = T / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, T));
-    return v;
-}
+DEV double tmin(double v) { return tred<T>(v, [](double a, double b) { return fmin(a, b); }); }
+template <int T>
+DEV double tmax(double v) { return tred<T>(v, [](double a, double b) { return fmax(a, b); }); }
+
+// lane k of the team (k uniform across the team, not compile-time)
 template <int T>
 DEV double bc(double v, int k) { return __shfl(v, k, T); }
 
-// component k of a slot array: lane k % T, slot k / T (k compile-time)
-#define BCK(arr, k) bc<T>((arr)[(k) / T], (k) % T)
+// lane K of the team, K compile-time: a quad_perm broadcast for teams of 4,
+// else ds_swizzle in bit mode (source lane = (lane & ~(T-1)) | K within each
+// 32-lane half).
+template <int T, int K>
+DEV double bck(double v)
+{
+    if constexpr (T == 4) {
+        return dpp_d<K | (K << 2) | (K << 4) | (K << 6)>(v);
+    } else {
+        constexpr int pat = ((32 - T) & 0x1F) | (K << 5);
+        const long long u = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_ds_swizzle((int)(u & 0xffffffffll), pat);
+        const int hi = __builtin_amdgcn_ds_swizzle((int)(u >> 32), pat);
+        return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+}
+
+// f(std::integral_constant<int, k>) for k = 0 .. N-1, in order
+template <typename F, int... K>
+DEV void static_for_impl(F&& f, std::integer_sequence<int, K...>) { (f(std::integral_constant<int, K>{}), ...); }
+template <int N, typename F>
+DEV void static_for(F&& f) { static_for_impl(f, std::make_integer_sequence<int, N>{}); }
+
+// component k of a slot array: lane k % T, slot k / T (k a constant
+// expression: the loop variable of a static_for)
+#define BCK(arr, k) bck<T, (k) % T>((arr)[(k) / T])
 #define SLOTS UNROLL for (int c = 0; c < L; c++)
 
 // slot s of an array, s uniform across the team but not compile-time
@@ -129,10 +172,11 @@ DEV void mix(const LaneFluid<NC, T>& lf, const Lane<NC, T>& ln, const Cell<NC, T
     constexpr int L = Cell<NC, T>::L;
     double w[L], acc[L];
     SLOTS { w[c] = ln.act[c] ? C.sA[c] * x[c] : 0.0; acc[c] = 0; }
-    UNROLL for (int k = 0; k < NC; k++) {
+    static_for<NC>([&](auto kc_) {
+        constexpr int k = decltype(kc_)::value;
         const double wk = BCK(w, k);
         SLOTS acc[c] += lf.omk[c][k] * wk;
-    }
+    });
     SLOTS s[c] = C.sA[c] * acc[c];
     A = TSUM(x[c] * s[c]);
     B = TSUM(C.Bi[c] * x[c]);
@@ -190,6 +234,16 @@ DEV double lnfn_ik(double A, double B, double Z, double lG, double G, double P, 
     return 1 / C * Cn + Dn + En * lG + E / G * Gn;
 }
 
+// One butterfly stage of the pivot search: larger |value|, ties to the lower
+// row; a commutative choice, so the team agrees on the pivot.
+template <int CTRL>
+DEV void argmax_step(double& v, int& p)
+{
+    const double v2 = dpp_d<CTRL>(v);
+    const int p2 = dpp_i<CTRL>(p);
+    if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
+}
+
 // Solve M x = b with rows i = c*T + t on the team (partial pivoting, ties to
 // the lower row).  Returns x of the lane's components.
 template <int N, int T, int L>
@@ -208,11 +262,10 @@ DEV void team_solve(double (&M)[L][N], double (&b)[L], const bool (&act)[L], int
             const int rc = c * T + t;
             if (vc > v || (vc == v && rc < p)) { v = vc; p = rc; }
         }
-        UNROLL for (int o = T / 2; o > 0; o >>= 1) {
-            const double v2 = __shfl_xor(v, o, T);
-            const int p2 = __shfl_xor(p, o, T);
-            if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
-        }
+        argmax_step<0xB1>(v, p);
+        argmax_step<0x4E>(v, p);
+        if constexpr (T >= 8) argmax_step<0x141>(v, p);
+        if constexpr (T >= 16) argmax_step<0x140>(v, p);
         piv[col] = p;
         const int pl = p % T, ps = p / T;
         double colv[L];
@@ -552,7 +605,8 @@ DEV void stable_nr_pre(const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const Lane<NC
     // own rows: entries (i, k) for every k and the row sums t_i = sum_k y_k S_ik
     double row[L][NC], tr[L];
     SLOTS tr[c] = 0;
-    UNROLL for (int k = 0; k < NC; k++) {
+    static_for<NC>([&](auto kc_) {
+        constexpr int k = decltype(kc_)::value;
         const double sAk = BCK(C.sA, k), Bxk = BCK(C.Bi, k), Axk = BCK(Ax, k);
         const double Zxk = BCK(Zx, k), yk = BCK(C.y, k);
         SLOTS {
@@ -561,11 +615,12 @@ DEV void stable_nr_pre(const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const Lane<NC
                                 Axk, Bxk, Zxk);
             tr[c] += yk * row[c][k];
         }
-    }
+    });
     // entries above the diagonal come from row k's lower triangle:
     // S_ki = (lnfx(k, i) - t_k + 1) / Yt, evaluated here from k's quantities
     double M[L][NC];
-    UNROLL for (int k = 0; k < NC; k++) {
+    static_for<NC>([&](auto kc_) {
+        constexpr int k = decltype(kc_)::value;
         const double sAk = BCK(C.sA, k), Bxk = BCK(C.Bi, k), Axk = BCK(Ax, k), yk = BCK(C.y, k);
         const double tk = BCK(tr, k);
         SLOTS {
@@ -576,7 +631,7 @@ DEV void stable_nr_pre(const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const Lane<NC
                                           Zx[c]) - tk + 1) / C.Yt;
             M[c][k] = (k <= i) ? own : other;
         }
-    }
+    });
     double rhs[L], sol[L];
     SLOTS rhs[c] = C.a3[c];
     team_solve<NC, T, L>(M, rhs, ln.act, ln.t, sol);
@@ -608,7 +663,8 @@ DEV void split_nr_pre(const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const Lane<NC,
         Zn1[c] = ((B1 - Z1) * An1[c] + bc1 * Bn1[c]) / den1;
     }
     double J[L][NC];
-    UNROLL for (int k = 0; k < NC; k++) {
+    static_for<NC>([&](auto kc_) {
+        constexpr int k = decltype(kc_)::value;
         const double sAk = BCK(C.sA, k), Bik = BCK(C.Bi, k);
         const double An0k = BCK(An0, k), Bn0k = BCK(Bn0, k), Zn0k = BCK(Zn0, k);
         const double An1k = BCK(An1, k), Bn1k = BCK(Bn1, k), Zn1k = BCK(Zn1, k);
@@ -628,17 +684,18 @@ DEV void split_nr_pre(const LaneFluid<NC, T>& lf, Cell<NC, T>& C, const Lane<NC,
                                          An1[c], Bn1[c], Zn1[c]);
             J[c][k] = (k <= i) ? own : other;
         }
-    }
+    });
     C.cnt[6] += 2;
     C.cnt[7]++;
     double rhs[L], dn[L];
     SLOTS rhs[c] = C.a5[c];
     team_solve<NC, T, L>(J, rhs, ln.act, ln.t, dn);
     double alpha = 1;
-    UNROLL for (int i = 0; i < NC; i++) {       // components in order
+    static_for<NC>([&](auto kc_) {
+        constexpr int i = decltype(kc_)::value;
         const double a1i = BCK(C.a1, i), dni = BCK(dn, i);
         if (a1i + alpha * dni < 0) alpha *= 0.9 * fabs(a1i / dni);
-    }
+    });
     double d[L];
     SLOTS {
         d[c] = alpha * dn[c];
