@@ -135,7 +135,33 @@ def cpu_baseline(wl, ncell, dt):
             "sample": f"first {ncell} cells of the same workload, one full pass, {sec:.1f} s"}
 
 
-def assembly_measure(spec, steps, warmup, sync, dt):
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+
+
+def traffic_key(a):
+    """The bench configuration a committed traffic measurement must match."""
+    return f"cells={a.cells} Nc={a.Nc} seed={a.seed} assembly={a.assembly}"
+
+
+def measured_traffic(kernel, key):
+    """HBM bytes per launch of `kernel` (a substring of its name) from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per
+    dispatch, scaled by the factors a plain copy gives), or None when that
+    summary was not taken on this configuration or lacks the kernel."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != key:
+        return None
+    for name, r in d.get("kernels", {}).items():
+        if kernel in name:
+            return r["hbm_bytes"]
+    return None
+
+
+def assembly_measure(spec, steps, warmup, sync, dt, key=None):
     from opencaeporox_amd import workload as W
     from opencaeporox_amd.fim import BATCHED, IsoTFIM
     from opencaeporox_amd.roofline import PEAK_HBM_GBS, assembly_bytes
@@ -154,8 +180,8 @@ def assembly_measure(spec, steps, warmup, sync, dt):
             "ms_per_pass": ms, "cells_per_s": nb / (ms * 1e-3), "kernel_ms": k,
             "assembly_conns_per_s": nconn / (k["asm"] * 1e-3),
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": gbs / PEAK_HBM_GBS, "traffic": None,
-                         "algorithmic_bytes": byts}}
+                         "frac": gbs / PEAK_HBM_GBS, "traffic": measured_traffic(f"res_asm<{nc}>", key),
+                         "kernel": f"res_asm<{nc}>", "algorithmic_bytes": byts}}
 
 
 def adjoint_measure(spec, steps, sync, dt):
@@ -274,6 +300,13 @@ def main():
                      "flops": flash_flops(it, a.Nc)}
         log(f"{name}: {ms:.2f} ms/step, {dev.nb / (ms * 1e-3):.3e} cells/s, kernels {k}")
     head = out.get("batched", out[modes[-1]])
+    lanes = dev.flash_mapping() if "batched" in out else 1
+    if "batched" not in out:
+        pe_kernel = f"pe_plain<{a.Nc}>"
+    elif lanes > 1:
+        pe_kernel = f"pe_team_batched<{a.Nc}, {lanes}, false>"
+    else:
+        pe_kernel = f"pe_batched<{a.Nc}>"
     tflops = head["flops"] / (head["kernel_ms"]["pe"] * 1e-3) / 1e12
     line = {
         "metric": METRIC, "value": head["cells_per_s"], "unit": "cells/s", "n_gpus": 1,
@@ -286,9 +319,8 @@ def main():
                    "lanes_per_cell": dev.flash_mapping() if "batched" in out else 1,
                    "autotune_ms": tune},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": tflops / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": ("pe_team_batched" if dev.flash_mapping() > 1 else "pe_batched")
-                               if "batched" in out else "pe_plain",
+                     "frac": tflops / PEAK_FP64_TFLOPS, "traffic": measured_traffic(pe_kernel, traffic_key(a)),
+                     "kernel": pe_kernel,
                      "flops_per_launch": head["flops"], "kernel_ms": head["kernel_ms"]["pe"]},
     }
     if "plain" in out and "batched" in out:
@@ -298,7 +330,7 @@ def main():
         line["kernel_ms"] = out["batched"]["kernel_ms"]
     dev.close()
     if a.assembly:
-        line["assembly"] = assembly_measure(a.assembly, max(3, a.steps // 2), 1, sync, dt)
+        line["assembly"] = assembly_measure(a.assembly, max(3, a.steps // 2), 1, sync, dt, traffic_key(a))
         log(f"assembly: {line['assembly']['kernel_ms']}")
     if a.multimodel:
         line["multimodel"] = multimodel_measure(a.multimodel, 3, sync, dt)

@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, co
     UNROLL for (int k = 0; k < np; k++) dS[k] = X[k * B + r];
     double* A = c.aux + (size_t)n * RL;
     const double one = r == 0 ? 1.0 : 0.0;
+    const double depth = c.depth[n];
     UNROLL for (int j = 0; j < np; j++) {
         const bool ex = c.phaseExist[j * nb + n] != 0;
         double drho = 0, dxi = 0, dmu = 0;
@@ -149,16 +150,10 @@ __global__ void __launch_bounds__(64) krpc_aux(const DevSat* __restrict__ sp, co
         A[SV + 3 * B * j + B + r] = dPj;
         A[SV + 3 * B * j + 2 * B + r] = drho;
         if (r < nc) A[SX + j * nc + r] = c.xij[(size_t)(j * nc + r) * nb + n];
-        if (r == 0) {
-            double* Sj = A + 8 * j;
-            Sj[0] = c.rho[j * nb + n];
-            Sj[1] = P + pc[j];
-            Sj[2] = xi;
-            Sj[3] = kr[j];
-            Sj[4] = mu;
-            Sj[5] = ex ? 1.0 : 0.0;
-            Sj[6] = lam;
-            Sj[7] = c.depth[n];
+        if (r < 4) {                          // the phase's 8 scalars, one 16-byte pair per lane
+            const double v0 = r == 0 ? c.rho[j * nb + n] : (r == 1 ? xi : (r == 2 ? mu : lam));
+            const double v1 = r == 0 ? P + pc[j] : (r == 1 ? kr[j] : (r == 2 ? (ex ? 1.0 : 0.0) : depth));
+            reinterpret_cast<double2*>(A + 8 * j)[r] = make_double2(v0, v1);
         }
     }
 }
@@ -342,19 +337,21 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
             }
             if (r > 0) res += ownB ? dt * flux : -dt * flux;
             double* off = bsr.val + (size_t)s * B * B + r;        // column r of the slot's block
-            off[0] = ownB ? dt * 0.0 : -(dt * 0.0);               // the volume row carries no flux
+            // the volume row carries no flux; blocks are written once and not
+            // read again here, so they bypass the caches' reuse (nontemporal)
+            __builtin_nontemporal_store(ownB ? dt * 0.0 : -(dt * 0.0), &off[0]);
             UNROLL for (int i = 0; i < nc; i++) {
                 if (ownB) {                       // AddDiag(b, B); NewOffDiag(b, e, E)
                     diag[i + 1] += dt * rb[i];
-                    off[(i + 1) * B] = dt * re[i];
+                    __builtin_nontemporal_store(dt * re[i], &off[(i + 1) * B]);
                 } else {                          // NewOffDiag(e, b, -B); AddDiag(e, -E)
-                    off[(i + 1) * B] = -(dt * rb[i]);
+                    __builtin_nontemporal_store(-(dt * rb[i]), &off[(i + 1) * B]);
                     diag[i + 1] += -(dt * re[i]);
                 }
             }
         }
         double* dg = bsr.val + (size_t)s0 * B * B + r;
-        UNROLL for (int i = 0; i < B; i++) dg[i * B] = diag[i];
+        UNROLL for (int i = 0; i < B; i++) __builtin_nontemporal_store(diag[i], &dg[i * B]);
         const double out = -res;             // Dscalar(-1) :1170
         resAbs[(size_t)n * B + r] = out;
         if (bsr.rhs) bsr.rhs[(size_t)n * B + r] = out;   // CopyRhs

@@ -158,9 +158,9 @@ OCPX_HD void eos_lnphi(const EosCell<NC>& e, const EosMix<NC>& m, double* lnphi)
     const double lz = log(Z - B);
     const double lg = log((Z + D1 * B) / (Z + D2 * B));
     const double cA = A / (D1 - D2) / B * lg;
-    const double iB = 1 / B;
+    const double iB = 1 / B, iA = 1 / A;
     UNROLL for (int i = 0; i < NC; i++)
-        lnphi[i] = e.Bi[i] * iB * (Z - 1) - lz - cA * (2 * m.s[i] / A - e.Bi[i] * iB);
+        lnphi[i] = e.Bi[i] * iB * (Z - 1) - lz - cA * (2 * m.s[i] * iA - e.Bi[i] * iB);
 }
 
 // Shared pieces of the Z derivatives (src/OCPEoS.cpp:135-180)

@@ -346,8 +346,8 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
         normalise_y(L);
         break;
     }
-    case ST_T_SSM:   // Y = di / phiSta (:439-444)
-        UNROLL for (int i = 0; i < NC; i++) L.y[i] = L.a1[i] / L.a2[i];
+    case ST_T_SSM:   // Y = di / phiSta (:439-444), a2 = ln phiSta
+        UNROLL for (int i = 0; i < NC; i++) L.y[i] = L.a1[i] * exp(-L.a2[i]);
         normalise_y(L);
         break;
     case ST_A_ITER: {   // Y = x ks, dY against the previous Y (:330-346)
@@ -466,13 +466,15 @@ OCPX_HD void pe_step(const DevFluid& f, const PeIO& io, PeLane<NC>& L)
         break;
     case ST_T_START:
     case ST_T_SSM: {   // Se = sum ln(fugSta / fugId * Yt)^2 (:429-432, :448-451)
+        // After an SSM update Y Yt = di / phiSta of the previous iterate, so
+        // ln(fugSta / fugId * Yt) is the change of ln phiSta.
         double Se = 0;
+        const bool first = L.st == ST_T_START;
         UNROLL for (int i = 0; i < NC; i++) {
-            const double phS = exp(ev[i]);
-            const double r = log((phS * L.y[i] * L.P) / (L.a1[i] * L.P) * L.Yt);
+            const double r = first ? log((exp(ev[i]) * L.y[i] * L.P) / (L.a1[i] * L.P) * L.Yt) : ev[i] - L.a2[i];
             Se += r * r;
             L.a3[i] = -r;                          // resSTA for StableNR
-            L.a2[i] = phS;
+            L.a2[i] = ev[i];
         }
         if (L.st == ST_T_START) {
             L.iter = 0;

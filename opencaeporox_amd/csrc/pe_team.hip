@@ -188,8 +188,8 @@ DEV double lnphi_of(double s, double Bi, double A, double B, double Z)
     const double lz = log(Z - B);
     const double lg = log((Z + D1 * B) / (Z + D2 * B));
     const double cA = A / (D1 - D2) / B * lg;
-    const double iB = 1 / B;
-    return Bi * iB * (Z - 1) - lz - cA * (2 * s / A - Bi * iB);
+    const double iB = 1 / B, iA = 1 / A;
+    return Bi * iB * (Z - 1) - lz - cA * (2 * s * iA - Bi * iB);
 }
 
 DEV double zden(double A, double B, double Z)
@@ -726,8 +726,8 @@ DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cel
         normalise_y(C, ln);
         break;
     }
-    case ST_T_SSM:
-        SLOTS C.y[c] = ln.act[c] ? C.a1[c] / C.a2[c] : 0.0;
+    case ST_T_SSM:   // Y = di / phiSta with a2 = ln phiSta
+        SLOTS C.y[c] = ln.act[c] ? C.a1[c] * exp(-C.a2[c]) : 0.0;
         normalise_y(C, ln);
         break;
     case ST_A_ITER: {
@@ -783,12 +783,17 @@ DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cel
         break;
     case ST_T_START:
     case ST_T_SSM: {
+        // r = ln(fugSta / fugId * Yt).  After an SSM update Y Yt = di / phiSta
+        // of the previous iterate, so r is the change of ln phiSta.
         double r[L];
+        if (C.st == ST_T_START) {
+            SLOTS r[c] = ln.act[c] ? log((exp(ev[c]) * C.y[c] * C.P) / (C.a1[c] * C.P) * C.Yt) : 0.0;
+        } else {
+            SLOTS r[c] = ln.act[c] ? ev[c] - C.a2[c] : 0.0;
+        }
         SLOTS {
-            const double phS = exp(ev[c]);
-            r[c] = ln.act[c] ? log((phS * C.y[c] * C.P) / (C.a1[c] * C.P) * C.Yt) : 0.0;
             C.a3[c] = -r[c];
-            C.a2[c] = ln.act[c] ? phS : 0.0;
+            C.a2[c] = ln.act[c] ? ev[c] : 0.0;
         }
         const double Se = TSUM(r[c] * r[c]);
         if (C.st == ST_T_START) {
@@ -840,7 +845,7 @@ DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cel
         C.cnt[2]++;
         C.iter++;
         double q[L], Kn[L];
-        SLOTS {
+        SLOTS {   // literal forms of :629-635 (overflow behaviour included)
             const double phi0 = exp(C.a5[c]), phi1 = exp(ev[c]);
             q[c] = ln.act[c] ? (phi1 * C.a3[c] * C.P) / (phi0 * C.a2[c] * C.P) - 1 : 0.0;
             Kn[c] = ln.act[c] ? phi1 / phi0 : 0.0;
