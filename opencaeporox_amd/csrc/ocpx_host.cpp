@@ -337,9 +337,11 @@ int launch_krpc_rock(ocpx_ctx* c)
 
 int launch_res_asm(ocpx_ctx* c, double dt, bool dr, bool dj)
 {
-    TimerScope t(c, dj ? 4 : 3);
-    HIPCHK(c, run_res_asm(c->nc, cell_io(c), conn_io(c), bsr_io(c), dt, dr, dj, c->resAbs, c->bmax, c->barg,
-                          c->st));
+    {
+        TimerScope t(c, dj ? 4 : 3);       // the residual / assembly kernel alone
+        HIPCHK(c, run_res_asm(c->nc, cell_io(c), conn_io(c), bsr_io(c), dt, dr, dj, c->resAbs, c->bmax, c->barg,
+                              c->st));
+    }
     if (dr) HIPCHK(c, run_norms_final(res_asm_blocks(c->nc, c->nb), c->bmax, c->barg, c->norms4, c->st));
     return OCPX_OK;
 }
