@@ -49,6 +49,13 @@ def parse():
                    help="models of the uneven multi-model batch (256 = the full batch; 0 = skip)")
     p.add_argument("--adjoint", type=str, default="60x220x85",
                    help="grid of the adjoint (J^T lambda) measurement, '' to skip")
+    p.add_argument("--cpu-procs", type=int, default=0,
+                   help="host threads for the parallel CPU baseline, one contiguous cell block "
+                        "each (0 = min(16, cores))")
+    p.add_argument("--assembly-b7", type=int, default=1,
+                   help="also measure the assembly at b = 7 (Nc = 5) on the same grid")
+    p.add_argument("--peaks", type=int, default=1,
+                   help="measure achievable copy bandwidth and FP64 DGEMM rate on this GPU")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only copy a buffer of this many MiB on the GPU (calibrates the "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -111,28 +118,95 @@ def time_steps(dev, mode, dt, steps, warmup, sync):
     return (t1 - t0) * 1e3 / steps, k
 
 
-def cpu_baseline(wl, ncell, dt):
-    """The oracle (the C restatement of the reference loops, one host core)
-    on the first ncell cells of the same workload: flash + kr/Pc + rock +
-    residual + assembly."""
+def _oracle_block(wl, lo, hi):
+    """Oracle state and a connection-free grid of cells lo..hi-1."""
     from opencaeporox_amd import workload as W
-    from oracle.oracle import Oracle, OracleState
-    o = Oracle()
-    nb = wl.grid.nb
-    g = wl.grid
-    sub = W.Grid(ncell, 1, 1, g.v[:ncell], g.depth[:ncell], g.poro[:ncell],
+    from oracle.oracle import OracleState
+    nb, g, n = wl.grid.nb, wl.grid, hi - lo
+    sub = W.Grid(n, 1, 1, g.v[lo:hi], g.depth[lo:hi], g.poro[lo:hi],
                  np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0))
-    init = {k: np.asarray(v).reshape(nb, -1)[:ncell].ravel() for k, v in wl.state.items()}
-    st = OracleState(wl.fluid.nc, ncell, sub.v, sub.poro, sub.depth, init=init)
-    t = time.perf_counter()
+    init = {k: np.asarray(v).reshape(nb, -1)[lo:hi].ravel() for k, v in wl.state.items()}
+    return sub, OracleState(wl.fluid.nc, n, sub.v, sub.poro, sub.depth, init=init)
+
+
+def _oracle_pass(o, wl, sub, st, dt):
     o.cal_flash(wl.fluid, st)
     o.cal_krpc(wl.sat, st)
     o.cal_rock(wl.rock, st)
     o.cal_res(wl.fluid, st, sub, dt)
     o.assemble(wl.fluid, st, sub, dt)
-    sec = time.perf_counter() - t
-    return {"value": ncell / sec, "unit": "cells/s", "cores": 1, "kind": "port",
-            "sample": f"first {ncell} cells of the same workload, one full pass, {sec:.1f} s"}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, ncell, dt, procs):
+    """The oracle (the C restatement of the reference loops) on the same
+    workload: flash + kr/Pc + rock + residual + assembly.  MPI-rank style:
+    `procs` host threads, each on its own contiguous block of cells (the C
+    calls release the GIL and share no state), timed together; and one core
+    on the first `ncell` cells."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.oracle import Oracle
+    o = Oracle()
+    sub, st = _oracle_block(wl, 0, ncell)
+    t = time.perf_counter()
+    _oracle_pass(o, wl, sub, st, dt)
+    sec1 = time.perf_counter() - t
+    one = {"value": ncell / sec1, "cores": 1, "sample": f"first {ncell} cells, one full pass, {sec1:.1f} s"}
+    per = max(1, ncell // 2)
+    n = min(wl.grid.nb, per * procs)
+    edges = np.linspace(0, n, procs + 1).astype(int)
+    blocks = [_oracle_block(wl, edges[k], edges[k + 1]) for k in range(procs)]
+    with ThreadPoolExecutor(procs) as ex:
+        t = time.perf_counter()
+        list(ex.map(lambda b: _oracle_pass(o, wl, b[0], b[1], dt), blocks))
+        sec = time.perf_counter() - t
+    return {"value": n / sec, "unit": "cells/s", "cores": procs, "kind": "port",
+            "sample": f"first {n} cells of the same workload in {procs} contiguous blocks, one thread "
+                      f"each (MPI-rank style), one full pass, {sec:.1f} s",
+            "cpu": cpu_model(), "single_core": one}
+
+
+def measured_peaks(sync):
+    """Achievable rates on this GPU, beside the published peaks: a 4 GiB
+    device copy (read + write bytes) and an FP64 8192^3 GEMM (2 n^3 flops)."""
+    import torch
+    x = torch.ones(1 << 29, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    y.copy_(x)
+    sync()
+    e0.record()
+    for _ in range(5):
+        y.copy_(x)
+    e1.record()
+    sync()
+    gbs = 5 * 2 * x.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del x, y
+    n = 8192
+    a = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    b = torch.randn(n, n, dtype=torch.float64, device="cuda")
+    c = a @ b
+    sync()
+    e0.record()
+    for _ in range(3):
+        c = a @ b
+    e1.record()
+    sync()
+    tf = 3 * 2 * n ** 3 / (e0.elapsed_time(e1) * 1e-3) / 1e12
+    del a, b, c
+    torch.cuda.empty_cache()
+    return {"copy_GBs": gbs, "dgemm_fp64_TFLOPs": tf,
+            "method": "torch copy_ of 4 GiB (2 x bytes), torch matmul FP64 8192^3 (2 n^3 flops)"}
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
@@ -161,12 +235,12 @@ def measured_traffic(kernel, key):
     return None
 
 
-def assembly_measure(spec, steps, warmup, sync, dt, key=None):
+def assembly_measure(spec, steps, warmup, sync, dt, key=None, Nc=6, solve=True):
     from opencaeporox_amd import workload as W
     from opencaeporox_amd.fim import BATCHED, IsoTFIM
     from opencaeporox_amd.roofline import PEAK_HBM_GBS, assembly_bytes
     nx, ny, nz = (int(v) for v in spec.split("x"))
-    wl = W.config3(nx, ny, nz, Nc=6)
+    wl = W.config3(nx, ny, nz, Nc=Nc)
     dev = IsoTFIM.from_workload(wl)
     dev.UpdateLastTimeStep()
     dev.autotune_flash()
@@ -175,13 +249,40 @@ def assembly_measure(spec, steps, warmup, sync, dt, key=None):
     nb, nconn, nc = dev.nb, dev.nconn, dev.nc
     byts = assembly_bytes(nb, nconn, nc)
     gbs = byts / (k["asm"] * 1e-3) / 1e9
+    out = {"workload": wl.name, "cells": nb, "conns": nconn, "block": nc + 1,
+           "ms_per_pass": ms, "cells_per_s": nb / (ms * 1e-3), "kernel_ms": k,
+           "assembly_conns_per_s": nconn / (k["asm"] * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": gbs / PEAK_HBM_GBS, "traffic": measured_traffic(f"res_asm<{nc}>", key),
+                        "kernel": f"res_asm<{nc}>", "algorithmic_bytes": byts}}
+    if solve:
+        out["linear_solve"] = solve_measure(dev, sync)
     dev.close()
-    return {"workload": wl.name, "cells": nb, "conns": nconn, "block": nc + 1,
-            "ms_per_pass": ms, "cells_per_s": nb / (ms * 1e-3), "kernel_ms": k,
-            "assembly_conns_per_s": nconn / (k["asm"] * 1e-3),
-            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": gbs / PEAK_HBM_GBS, "traffic": measured_traffic(f"res_asm<{nc}>", key),
-                         "kernel": f"res_asm<{nc}>", "algorithmic_bytes": byts}}
+    return out
+
+
+def solve_measure(dev, sync, rtol=1e-3, maxit=200):
+    """Device BiCGStab + block Jacobi on the assembled system, with the
+    reference's solver settings (1e-3 on ||r|| / ||b||, 200 iterations), and
+    the HBM roofline of its BSR matrix-vector product."""
+    from opencaeporox_amd.roofline import PEAK_HBM_GBS
+    dev.LinearSolve(rtol, maxit, fetch=False)        # allocates the Krylov vectors
+    sync()
+    dev.timing(True)
+    _, it, rr = dev.LinearSolve(rtol, maxit, fetch=False)
+    sync()
+    total = dev.timing_get(7)[0]
+    spmv, calls = dev.timing_get(8)
+    dev.timing(False)
+    nb, B = dev.nb, dev.bs
+    nnzb = nb + 2 * dev.nconn
+    byts = nnzb * (8 * B * B + 4) + 8 * (nb + 1) + 16 * nb * B     # blocks, columns, rowptr, x and y once
+    gbs = byts / (spmv * 1e-3) / 1e9
+    return {"method": "BiCGStab, block-Jacobi preconditioner", "rtol": rtol, "maxit": maxit, "iterations": it,
+            "relres": rr, "ms": total, "spmv_calls": calls, "spmv_ms": spmv,
+            "spmv_roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                              "frac": gbs / PEAK_HBM_GBS, "kernel": f"bsr_spmv<{B}>",
+                              "algorithmic_bytes": byts}}
 
 
 def adjoint_measure(spec, steps, sync, dt):
@@ -332,14 +433,27 @@ def main():
     if a.assembly:
         line["assembly"] = assembly_measure(a.assembly, max(3, a.steps // 2), 1, sync, dt, traffic_key(a))
         log(f"assembly: {line['assembly']['kernel_ms']}")
+        if a.assembly_b7:
+            line["assembly_b7"] = assembly_measure(a.assembly, max(3, a.steps // 2), 1, sync, dt, traffic_key(a),
+                                                   Nc=5)
+            log(f"assembly b=7: {line['assembly_b7']['kernel_ms']}")
     if a.multimodel:
         line["multimodel"] = multimodel_measure(a.multimodel, 3, sync, dt)
         log(f"multi-model: {line['multimodel']['cells_per_s']:.3e} cells/s")
     if a.adjoint:
         line["adjoint"] = adjoint_measure(a.adjoint, max(3, a.steps // 2), sync, dt)
         log(f"adjoint: {line['adjoint']['kernel_ms']}")
+    if a.peaks:
+        pk = measured_peaks(sync)
+        line["measured_peaks"] = pk
+        line["roofline"]["frac_of_measured"] = line["roofline"]["achieved"] / pk["dgemm_fp64_TFLOPs"]
+        if "assembly" in line:
+            r = line["assembly"]["roofline"]
+            r["frac_of_measured"] = r["achieved"] / pk["copy_GBs"]
+        log(f"measured peaks: {pk}")
     if a.cpu_cells > 0:
-        line["cpu_baseline"] = cpu_baseline(wl, min(a.cpu_cells, wl.grid.nb), dt)
+        procs = a.cpu_procs or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(wl, min(a.cpu_cells, wl.grid.nb), dt, procs)
         log(f"cpu baseline: {line['cpu_baseline']}")
     print(json.dumps(line), flush=True)
 

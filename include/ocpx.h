@@ -255,8 +255,22 @@ int  ocpx_bsr_device_ptrs(ocpx_ctx* ctx, void** rowptr, void** colind,
 
 /* ≙ IsoT_FIM::GetSolution (src/IsoThermalMethod.cpp:1282-1418) without
  * wells or halos: dSmax chop, S/xij/P/Ni update.  u is a host array
- * [nb*(nc+1)]. */
+ * [nb*(nc+1)], or NULL for the device solution of the last
+ * ocpx_linear_solve (no copy). */
 int  ocpx_get_solution(ocpx_ctx* ctx, const double* u, double dSmax);
+
+/* ≙ LinearSystem::AssembleMatLinearSolver + Solve (src/LinearSystem.cpp:96-119)
+ * as IsoT_FIM::SolveLinearSystem calls them (src/IsoThermalMethod.cpp:754-
+ * 768), on the device BSR of ocpx_assemble: J u = rhs (rhs = -R, CopyRhs).
+ * BiCGStab with a block-Jacobi preconditioner (inverse of each diagonal
+ * block) in place of the reference's external FASP/PETSc solvers; stops at
+ * ||r|| / ||b|| < rtol (the reference's stop_type 1; its decks use 1e-3 and
+ * 200 iterations, data/spe5/bsr.fasp).  *iters = iterations, negative when
+ * not converged (the reference's convention); *relres = final ||r||/||b||.
+ * u (host [nb*(nc+1)], may be NULL) receives the solution, which also stays
+ * on the device for ocpx_get_solution(ctx, NULL, dSmax).  Synchronous. */
+int  ocpx_linear_solve(ocpx_ctx* ctx, double rtol, int32_t maxit, double* u, int32_t* iters,
+                       double* relres);
 
 /* ≙ IsoT_FIM::UpdateLastTimeStep (src/IsoThermalMethod.cpp:1473-1515) with
  * SkipPSAVarset::UpdateLastTimeStep (src/AcceleratePEC.cpp:24-31): keeps a
@@ -303,7 +317,8 @@ int  ocpx_dR_dparams_dev(ocpx_ctx* ctx, double dt, const double* lambda_d,
 /* Kernel timing: average duration (ms) of the kernels of one entry point
  * over the calls since the last reset, from HIP events recorded on the
  * context stream.  which: 0 PE, 1 flash tail, 2 krpc+rock, 3 residual,
- * 4 assembly, 5 J^T lambda, 6 dR/dparams. */
+ * 4 assembly, 5 J^T lambda, 6 dR/dparams, 7 linear solve (whole, host
+ * round trips included), 8 BSR matrix-vector product inside the solve. */
 int  ocpx_timing_enable(ocpx_ctx* ctx, int on);
 int  ocpx_timing_get(ocpx_ctx* ctx, int which, double* avg_ms, int64_t* calls);
 

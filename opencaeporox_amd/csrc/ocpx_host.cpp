@@ -75,6 +75,9 @@ struct ocpx_ctx {
     double* norms4 = nullptr;
     double *lam = nullptr, *out = nullptr, *gT = nullptr, *gV = nullptr;
     bool flashed = false, assembled = false;
+    // linear solve (allocated on the first ocpx_linear_solve)
+    double *sol = nullptr, *dinv = nullptr, *kv[8] = {}, *dpart = nullptr, *dout = nullptr;
+    bool solved = false;
     bool aux_valid = false;      // krpc_aux ran on the current flash state
     int grid_waves = 0;
     int pe_lanes = 1;            // batched-flash mapping: lanes per cell (1, 4, 8, 16)
@@ -82,7 +85,7 @@ struct ocpx_ctx {
     int team_blocks[3] = {0, 0, 0}; // persistent grid of the team kernels, T = 4 / 8 / 16
     // timing
     bool timing = false;
-    TimingAcc tacc[7];
+    TimingAcc tacc[9];
     // graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -822,14 +825,109 @@ int ocpx_bsr_device_ptrs(ocpx_ctx* c, void** rowptr, void** colind, void** val, 
 
 int ocpx_get_solution(ocpx_ctx* c, const double* u, double dSmax)
 {
-    if (!c || !u) return OCPX_E_ARG;
+    if (!c) return OCPX_E_ARG;
     if (!c->flashed) return fail(c, OCPX_E_STATE, "ocpx_get_solution needs dXsdXp from ocpx_flash_der");
     const size_t B = c->nc + 1;
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(c->lam, u, c->nb * B * 8, hipMemcpyHostToDevice));
+    const double* ud = c->sol;
+    if (u) {
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, hipMemcpy(c->lam, u, c->nb * B * 8, hipMemcpyHostToDevice));
+        ud = c->lam;
+    } else if (!c->solved) {
+        return fail(c, OCPX_E_STATE, "ocpx_get_solution(u = NULL) needs ocpx_linear_solve");
+    }
     HIPCHK(c, run_get_solution(c->nc, D(c, OCPX_F_P), D(c, OCPX_F_NI), D(c, OCPX_F_S), D(c, OCPX_F_XIJ),
-                               D(c, OCPX_F_DSEC_DPRI), c->lam, dSmax, c->nb, c->st));
+                               D(c, OCPX_F_DSEC_DPRI), ud, dSmax, c->nb, c->st));
     c->aux_valid = false;
+    c->solved = false;
+    return OCPX_OK;
+}
+
+// BiCGStab, right-preconditioned with block Jacobi, x0 = 0, stopping on
+// ||r|| / ||b|| (the reference's stop_type 1); scalars come back to the host
+// once per dot product.
+int ocpx_linear_solve(ocpx_ctx* c, double rtol, int32_t maxit, double* u, int32_t* iters, double* relres)
+{
+    if (!c || !(rtol > 0) || maxit <= 0) return OCPX_E_ARG;
+    if (!c->assembled) return fail(c, OCPX_E_STATE, "ocpx_linear_solve needs ocpx_assemble");
+    const int B = c->nc + 1;
+    const int64_t nb = c->nb, len = nb * B;
+    int rc = OCPX_OK;
+    if (!c->sol) {
+        if ((rc = dalloc(c, &c->sol, (size_t)len)) || (rc = dalloc(c, &c->dinv, (size_t)nb * B * B)) ||
+            (rc = dalloc(c, &c->dpart, (size_t)dot_scratch_len())) || (rc = dalloc(c, &c->dout, 1)))
+            return rc;
+        for (auto& v : c->kv)
+            if ((rc = dalloc(c, &v, (size_t)len))) return rc;
+    }
+    double *r = c->kv[0], *rh = c->kv[1], *p = c->kv[2], *ph = c->kv[3], *v = c->kv[4], *s = c->kv[5],
+           *sh = c->kv[6], *t = c->kv[7], *x = c->sol;
+    const BsrIO io = bsr_io(c);
+    hipStream_t st = c->st;
+    TimerScope whole(c, 7);
+    auto dot = [&](const double* a, const double* b, double& out) -> int {
+        HIPCHK(c, run_dot(a, b, len, c->dpart, c->dout, st));
+        HIPCHK(c, hipMemcpyAsync(&out, c->dout, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        return OCPX_OK;
+    };
+    auto spmv = [&](const double* in, double* outv) -> int {
+        TimerScope ts(c, 8);
+        HIPCHK(c, run_bsr_spmv(B, io.rowptr, io.colind, io.val, in, outv, nb, st));
+        return OCPX_OK;
+    };
+    auto precond = [&](const double* in, double* outv) -> int {
+        HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv, in, outv, nb, st));
+        return OCPX_OK;
+    };
+    HIPCHK(c, run_bsr_diag_inv(B, io.rowptr, io.val, nb, c->dinv, st));
+    HIPCHK(c, run_axpbypcz(x, nullptr, 0, nullptr, 0, nullptr, len, st));
+    HIPCHK(c, run_axpbypcz(r, io.rhs, 0, nullptr, 0, nullptr, len, st));
+    HIPCHK(c, run_axpbypcz(rh, io.rhs, 0, nullptr, 0, nullptr, len, st));
+    HIPCHK(c, run_axpbypcz(p, nullptr, 0, nullptr, 0, nullptr, len, st));
+    HIPCHK(c, run_axpbypcz(v, nullptr, 0, nullptr, 0, nullptr, len, st));
+    double bb = 0;
+    if ((rc = dot(io.rhs, io.rhs, bb))) return rc;
+    const double bnorm = std::sqrt(bb);
+    int it = 0;
+    bool conv = bnorm == 0;
+    double res = 0, rho = 1, alpha = 1, omega = 1;
+    while (!conv && it < maxit && std::isfinite(bnorm)) {
+        it++;
+        double rho1 = 0, rv = 0, ss = 0, tt = 0, ts = 0, rr = 0;
+        if ((rc = dot(rh, r, rho1))) return rc;
+        if (rho1 == 0 || !std::isfinite(rho1)) break;                     // breakdown
+        const double beta = (rho1 / rho) * (alpha / omega);
+        HIPCHK(c, run_axpbypcz(p, r, beta, p, -beta * omega, v, len, st));  // p = r + beta (p - omega v)
+        if ((rc = precond(p, ph)) || (rc = spmv(ph, v)) || (rc = dot(rh, v, rv))) return rc;
+        if (rv == 0 || !std::isfinite(rv)) break;
+        alpha = rho1 / rv;
+        HIPCHK(c, run_axpbypcz(s, r, -alpha, v, 0, nullptr, len, st));    // s = r - alpha v
+        if ((rc = dot(s, s, ss))) return rc;
+        if (std::sqrt(ss) / bnorm < rtol) {
+            HIPCHK(c, run_axpbypcz(x, x, alpha, ph, 0, nullptr, len, st));
+            res = std::sqrt(ss) / bnorm;
+            conv = true;
+            break;
+        }
+        if ((rc = precond(s, sh)) || (rc = spmv(sh, t)) || (rc = dot(t, t, tt)) || (rc = dot(t, s, ts))) return rc;
+        if (tt == 0 || !std::isfinite(tt)) break;
+        omega = ts / tt;
+        HIPCHK(c, run_axpbypcz(x, x, alpha, ph, omega, sh, len, st));    // x += alpha ph + omega sh
+        HIPCHK(c, run_axpbypcz(r, s, -omega, t, 0, nullptr, len, st));   // r = s - omega t
+        if ((rc = dot(r, r, rr))) return rc;
+        res = std::sqrt(rr) / bnorm;
+        if (res < rtol) { conv = true; break; }
+        if (omega == 0 || !std::isfinite(res)) break;
+        rho = rho1;
+    }
+    if (iters) *iters = conv ? it : -std::max(it, 1);        // negative on failure, as LinearSystem::Solve
+    if (relres) *relres = res;
+    if (u) {
+        HIPCHK(c, hipMemcpyAsync(u, x, (size_t)len * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+    }
+    c->solved = true;
     return OCPX_OK;
 }
 
@@ -991,7 +1089,7 @@ int ocpx_timing_enable(ocpx_ctx* c, int on)
 
 int ocpx_timing_get(ocpx_ctx* c, int which, double* avg_ms, int64_t* calls)
 {
-    if (!c || which < 0 || which > 6) return OCPX_E_ARG;
+    if (!c || which < 0 || which > 8) return OCPX_E_ARG;
     TimingAcc& t = c->tacc[which];
     for (auto& p : t.pending) {
         HIPCHK(c, hipEventSynchronize(p.second));

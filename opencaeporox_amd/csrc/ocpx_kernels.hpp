@@ -118,6 +118,16 @@ hipError_t run_norms_final(int64_t nblocks, const double* blockmax, const int64_
 hipError_t run_get_solution(int nc, double* P, double* Ni, double* S, double* xij,
                             const double* dXsdXp, const double* u, double dSmax, int64_t nb,
                             hipStream_t st);
+// linear solve (solver.hip)
+hipError_t run_bsr_diag_inv(int B, const int64_t* rowptr, const double* val, int64_t nb, double* dinv,
+                            hipStream_t st);
+hipError_t run_bsr_spmv(int B, const int64_t* rowptr, const int32_t* col, const double* val, const double* x,
+                        double* y, int64_t nb, hipStream_t st);
+hipError_t run_dot(const double* x, const double* y, int64_t len, double* part, double* out, hipStream_t st);
+int dot_scratch_len();
+hipError_t run_axpbypcz(double* z, const double* x, double a, const double* y, double b, const double* w,
+                        int64_t len, hipStream_t st);
+
 hipError_t run_jt_lambda(int nc, int64_t nb, const BsrIO& bsr, const double* lambda, double* out,
                          hipStream_t st);
 hipError_t run_dR_dtrans(int nc, const CellIO& c, const ConnIO& cn, double dt,

@@ -184,9 +184,23 @@ class IsoTFIM:
                                         val.ctypes.data, rhs.ctypes.data), "bsr")
         return rowptr, col, val.reshape(n, self.bs, self.bs), rhs
 
-    def GetSolution(self, u, dSmax=0.1):
+    def GetSolution(self, u=None, dSmax=0.1):
+        """Newton update from a host u, or (u None) from the device solution
+        of the last LinearSolve."""
+        if u is None:
+            self._chk(self.lib.ocpx_get_solution(self.h, None, dSmax), "get solution")
+            return
         u = np.ascontiguousarray(u, np.float64)
         self._chk(self.lib.ocpx_get_solution(self.h, u.ctypes.data, dSmax), "get solution")
+
+    def LinearSolve(self, rtol=1e-3, maxit=200, fetch=True):
+        """Device BiCGStab + block Jacobi on the assembled system (ocpx_linear_solve).
+        Returns (u or None, iterations (negative if not converged), ||r||/||b||)."""
+        u = np.zeros(self.nb * self.bs) if fetch else None
+        it, rr = C.c_int32(), C.c_double()
+        self._chk(self.lib.ocpx_linear_solve(self.h, rtol, maxit, None if u is None else u.ctypes.data,
+                                             C.byref(it), C.byref(rr)), "linear solve")
+        return u, it.value, rr.value
 
     def set_geometry(self, v=None, trans=None):
         va = None if v is None else np.ascontiguousarray(v, np.float64)
