@@ -397,8 +397,12 @@ def main():
             log(f"autotuner (ms per mapping, lanes per cell): {tune} -> {dev.flash_mapping()}")
         ms, k = time_steps(dev, mode, dt, a.steps, a.warmup, sync)
         it = dev.get("ITERS")
+        fe = it.reshape(-1, 8)[:, 5]
         out[name] = {"ms_per_step": ms, "cells_per_s": dev.nb / (ms * 1e-3), "kernel_ms": k,
-                     "flops": flash_flops(it, a.Nc)}
+                     "flops": flash_flops(it, a.Nc),
+                     "fugacity_evals_per_cell": {q: float(np.percentile(fe, p)) for q, p in
+                                                 (("p50", 50), ("p99", 99), ("p99.9", 99.9))}
+                                                | {"mean": float(fe.mean()), "max": int(fe.max())}}
         log(f"{name}: {ms:.2f} ms/step, {dev.nb / (ms * 1e-3):.3e} cells/s, kernels {k}")
     head = out.get("batched", out[modes[-1]])
     lanes = dev.flash_mapping() if "batched" in out else 1
@@ -415,9 +419,11 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": wl.name, "cells": dev.nb, "Nc": a.Nc, "near_saturation_cells": n_near,
+                   "fugacity_evals_per_cell": head["fugacity_evals_per_cell"],
                    "pass": "reset + flash_der + kr/Pc + rock + residual + assembly",
                    "flash": "batched" if "batched" in out else modes[-1],
                    "lanes_per_cell": dev.flash_mapping() if "batched" in out else 1,
+                   "queue_longest_first": dev.flash_queue_order() if "batched" in out else False,
                    "autotune_ms": tune},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": tflops / PEAK_FP64_TFLOPS, "traffic": measured_traffic(pe_kernel, traffic_key(a)),

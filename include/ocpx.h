@@ -228,10 +228,20 @@ int  ocpx_get_pe_result(ocpx_ctx* ctx, int32_t* NP, double* nu, double* x);
 int  ocpx_set_flash_mapping(ocpx_ctx* ctx, int lanes_per_cell);
 int  ocpx_get_flash_mapping(ocpx_ctx* ctx, int32_t* lanes_per_cell);
 /* Autotuner: runs the batched phase-equilibrium kernel on the current state
- * once per mapping built for Nc (1, 4, 8, 16 lanes per cell), keeps the
- * fastest as the context's mapping.  ms[4] (may be NULL) receives the kernel
- * times in that order, 0 where a mapping does not apply.  Synchronous. */
+ * once per mapping built for Nc (1, 4, 8, 16 lanes per cell) with the
+ * longest-first queue, keeps the fastest as the context's mapping, then runs
+ * that mapping with the cells in natural order and keeps the faster queue
+ * order.  ms[5] (may be NULL) receives the kernel times: the four mappings
+ * (0 where one does not apply), then the chosen one in natural order.
+ * Synchronous. */
 int  ocpx_autotune_flash(ocpx_ctx* ctx, double* ms);
+/* Queue order of the batched flash.  No counterpart in the reference (its
+ * flash loops over cells in order); same results either way.  1 (default):
+ * each batched flash hands out the cells longest first, by their fugacity-
+ * evaluation counts in the previous batched flash (ITERS), so cells with the
+ * same count share wavefronts and the longest start first; 0: natural order. */
+int  ocpx_set_flash_queue_order(ocpx_ctx* ctx, int longest_first);
+int  ocpx_get_flash_queue_order(ocpx_ctx* ctx, int32_t* longest_first);
 
 /* ≙ IsoT_FIM::CalKrPc (src/IsoThermalMethod.cpp:1073-1088) */
 int  ocpx_krpc_der(ocpx_ctx* ctx);

@@ -65,6 +65,9 @@ struct ocpx_ctx {
     int32_t* pe_NP = nullptr;
     double *pe_nu = nullptr, *pe_x = nullptr;
     unsigned* queue = nullptr;
+    int qmode = 1;               // batched flash queue: 1 longest first, 0 natural order
+    int32_t* qorder = nullptr;   // [nb] batched flash: cell at each queue position
+    unsigned* qhist = nullptr;   // counting-sort buckets for the next order
     // linear system
     int64_t* rowptr = nullptr;
     int32_t *colind = nullptr, *tslot = nullptr, *nbr_conn = nullptr;
@@ -246,16 +249,25 @@ int launch_pe(ocpx_ctx* c, int mode)
     io.ftype_out = (int32_t*)c->f[OCPX_F_FTYPE_OUT].d;
     io.iters = (int32_t*)c->f[OCPX_F_ITERS].d;
     io.queue = c->queue;
+    io.qorder = (mode == OCPX_FLASH_BATCHED && c->qmode) ? c->qorder : nullptr;
     if (mode == OCPX_FLASH_BATCHED) HIPCHK(c, run_fill_u32(c->queue, 0u, 1, c->st));
-    TimerScope t(c, 0);
     io.mid = c->mid;
     if (c->mid && (mode != OCPX_FLASH_BATCHED || c->pe_lanes == 1))
         return fail(c, OCPX_E_UNSUPPORTED, "contexts of several models flash in batched mode with a team mapping");
-    if (mode == OCPX_FLASH_BATCHED && c->pe_lanes > 1) {
-        const int T = c->pe_lanes;
-        HIPCHK(c, run_pe_team(c->NC, T, c->df, io, true, c->team_blocks[team_slot(T)], c->st));
-    } else {
-        HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
+    {
+        TimerScope t(c, 0);
+        if (mode == OCPX_FLASH_BATCHED && c->pe_lanes > 1) {
+            const int T = c->pe_lanes;
+            HIPCHK(c, run_pe_team(c->NC, T, c->df, io, true, c->team_blocks[team_slot(T)], c->st));
+        } else {
+            HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
+        }
+    }
+    if (mode == OCPX_FLASH_BATCHED) {
+        // the next batched flash takes the cells longest first, by this one's
+        // fugacity-evaluation counts (ITERS row 5)
+        HIPCHK(c, run_fill_u32(c->qhist, 0u, queue_order_buckets(), c->st));
+        HIPCHK(c, run_queue_order(io.iters + (size_t)5 * c->nb, c->nb, c->qhist, c->qorder, c->st));
     }
     return OCPX_OK;
 }
@@ -479,6 +491,16 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     ALLOC(c->cb, c->nconn); ALLOC(c->ce, c->nconn); ALLOC(c->trans, c->nconn);
     ALLOC(c->pe_NP, c->nb); ALLOC(c->pe_nu, c->nb); ALLOC(c->pe_x, (size_t)2 * c->NC * c->nb);
     ALLOC(c->queue, 4);
+    ALLOC(c->qorder, (size_t)c->nb);
+    ALLOC(c->qhist, (size_t)queue_order_buckets());
+    {
+        std::vector<int32_t> iota((size_t)c->nb);
+        for (int64_t k = 0; k < c->nb; k++) iota[(size_t)k] = (int32_t)k;
+        if (c->nb && hipMemcpy(c->qorder, iota.data(), iota.size() * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+            *out = c;
+            return fail(c, OCPX_E_HIP, "queue order upload");
+        }
+    }
     ALLOC(c->aux, (size_t)c->nb * aux_len(c->nc + 1, c->nc));
     // BSR in OCPMatrix push order: diagonal first, then per connection in
     // index order, row e gets (e, b) and row b gets (b, e)
@@ -719,14 +741,53 @@ int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
         t[q] = e;
         if (best_ms == 0 || e < best_ms) { best_ms = e; best = cand[q]; }
     }
+    // the best mapping once more with the cells in natural order: the
+    // longest-first queue stays only if it was faster
+    double nat = 0;
+    const int keepq = c->qmode;
+    if (!rc) {
+        c->pe_lanes = best;
+        c->qmode = 0;
+        for (int rep = 0; rep < 2 && !rc; rep++) {
+            hipEventRecord(a, c->st);
+            rc = launch_pe(c, OCPX_FLASH_BATCHED);
+            hipEventRecord(b, c->st);
+        }
+        if (!rc && hipEventSynchronize(b) != hipSuccess) rc = fail(c, OCPX_E_HIP, "autotune sync");
+        if (!rc) {
+            float e = 0;
+            hipEventElapsedTime(&e, a, b);
+            nat = e;
+        }
+    }
     hipEventDestroy(a);
     hipEventDestroy(b);
     c->timing = tm;
     c->pe_lanes = keep;
+    c->qmode = keepq;
     if (rc) return rc;
-    if (ms) for (int q = 0; q < 4; q++) ms[q] = t[q];
+    if (ms) {
+        for (int q = 0; q < 4; q++) ms[q] = t[q];
+        ms[4] = nat;
+    }
     c->pe_lanes_auto = best;
+    c->qmode = nat > 0 && nat < best_ms ? 0 : 1;
     return ocpx_set_flash_mapping(c, best);
+}
+
+int ocpx_set_flash_queue_order(ocpx_ctx* c, int longest_first)
+{
+    if (!c || (longest_first != 0 && longest_first != 1)) return OCPX_E_ARG;
+    if (longest_first != c->qmode) destroy_graph(c);
+    c->qmode = longest_first;
+    return OCPX_OK;
+}
+
+int ocpx_get_flash_queue_order(ocpx_ctx* c, int32_t* longest_first)
+{
+    if (!c || !longest_first) return OCPX_E_ARG;
+    *longest_first = c->qmode;
+    return OCPX_OK;
 }
 
 int ocpx_get_pe_result(ocpx_ctx* c, int32_t* NP, double* nu, double* x)

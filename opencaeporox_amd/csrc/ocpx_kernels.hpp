@@ -33,8 +33,15 @@ struct PeIO {
     int32_t* ftype_out;         // [nb]
     int32_t* iters;             // [8][nb] or null
     unsigned* queue;            // work-queue counter (batched mode)
+    const int32_t* qorder;      // [nb] cell handed out at each queue position (batched mode), or null
     const int32_t* mid;         // [nb] model of each cell (multi-model contexts), or null
 };
+
+// queue order for the next batched flash (pe.hip): cells by descending count
+// of fugacity evaluations in the last one; hist has queue_order_buckets()
+// entries, zeroed by the caller
+int queue_order_buckets();
+hipError_t run_queue_order(const int32_t* evals, int64_t nb, unsigned* hist, int32_t* order, hipStream_t st);
 
 // flash-derivative tail (tail.hip): properties, labels, dVf, dXsdXp,
 // SkipPSA next-step data and the PassFlashValue writes

@@ -625,7 +625,10 @@ __global__ void __launch_bounds__(256) pe_batched(const DevFluid* __restrict__ f
             base = __shfl(base, leader);
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-                if (idx < (unsigned)io.nb) { pe_begin<NC>(f, io, (int)idx, L); have = true; }
+                if (idx < (unsigned)io.nb) {
+                    pe_begin<NC>(f, io, io.qorder ? io.qorder[idx] : (int)idx, L);
+                    have = true;
+                }
                 else drained = true;
             }
         }
@@ -658,6 +661,62 @@ static hipError_t launch_pe(const DevFluid* f, const PeIO& io, int mode, int gri
         if (nblk > need) nblk = (int)need;
         hipLaunchKernelGGL(pe_batched<NC>, dim3(nblk), dim3(bs), 0, st, f, io);
     }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ queue order
+// Longest-first order for the next batched flash: a counting sort of the
+// cells by their fugacity-evaluation count in the last flash (counts at or
+// above QB - 1 share the first bucket).  Cells with equal counts sit next to
+// each other, so a wavefront takes cells that tend to follow the same path,
+// and the long cells start first instead of extending the tail.  The order
+// inside a bucket is the atomics' order; it only schedules work.
+constexpr int QB = 4096;
+
+__device__ __forceinline__ int qbucket(int32_t e) { return QB - 1 - (e < 0 ? 0 : (e > QB - 1 ? QB - 1 : e)); }
+
+__global__ void __launch_bounds__(256) qorder_hist(const int32_t* __restrict__ evals, int64_t nb, unsigned* hist)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) atomicAdd(&hist[qbucket(evals[i])], 1u);
+}
+
+// exclusive prefix sum of hist[QB] in place, one block of 256 threads
+__global__ void __launch_bounds__(256) qorder_scan(unsigned* hist)
+{
+    constexpr int PER = QB / 256;
+    __shared__ unsigned part[256];
+    const int t = threadIdx.x;
+    unsigned loc[PER], s = 0;
+    UNROLL for (int k = 0; k < PER; k++) { loc[k] = hist[t * PER + k]; s += loc[k]; }
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {          // inclusive scan of the partial sums
+        const unsigned v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned run = part[t] - s;
+    UNROLL for (int k = 0; k < PER; k++) { hist[t * PER + k] = run; run += loc[k]; }
+}
+
+__global__ void __launch_bounds__(256) qorder_scatter(const int32_t* __restrict__ evals, int64_t nb, unsigned* offs,
+                                                      int32_t* __restrict__ order)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) order[atomicAdd(&offs[qbucket(evals[i])], 1u)] = (int32_t)i;
+}
+
+int queue_order_buckets() { return QB; }
+
+hipError_t run_queue_order(const int32_t* evals, int64_t nb, unsigned* hist, int32_t* order, hipStream_t st)
+{
+    if (nb == 0) return hipSuccess;
+    const unsigned g = (unsigned)((nb + 255) / 256);
+    hipLaunchKernelGGL(qorder_hist, dim3(g), dim3(256), 0, st, evals, nb, hist);
+    hipLaunchKernelGGL(qorder_scan, dim3(1), dim3(256), 0, st, hist);
+    hipLaunchKernelGGL(qorder_scatter, dim3(g), dim3(256), 0, st, evals, nb, hist, order);
     return hipGetLastError();
 }
 

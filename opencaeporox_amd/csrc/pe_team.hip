@@ -950,8 +950,9 @@ __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lead) - 1ull));
                 if (idx < (unsigned)io.nb) {
-                    C.model = MULTI ? io.mid[idx] : 0;
-                    begin<NC, T>(fp[C.model], lf, io, (int)idx, C, ln);
+                    const int cell = io.qorder ? io.qorder[idx] : (int)idx;
+                    C.model = MULTI ? io.mid[cell] : 0;
+                    begin<NC, T>(fp[C.model], lf, io, cell, C, ln);
                     have = true;
                 }
                 else drained = true;

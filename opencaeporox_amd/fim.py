@@ -147,9 +147,20 @@ class IsoTFIM:
     def autotune_flash(self):
         """Times the batched phase-equilibrium kernel per mapping on the
         current state; keeps the fastest.  Returns {lanes_per_cell: ms}."""
-        ms = np.zeros(4)
+        ms = np.zeros(5)
         self._chk(self.lib.ocpx_autotune_flash(self.h, ms.ctypes.data), "autotune")
-        return {k: v for k, v in zip((1, 4, 8, 16), ms) if v > 0}
+        out = {k: v for k, v in zip((1, 4, 8, 16), ms[:4]) if v > 0}
+        if ms[4] > 0:
+            out["natural_order"] = ms[4]
+        return out
+
+    def set_flash_queue_order(self, longest_first):
+        self._chk(self.lib.ocpx_set_flash_queue_order(self.h, 1 if longest_first else 0), "queue order")
+
+    def flash_queue_order(self):
+        v = C.c_int32()
+        self._chk(self.lib.ocpx_get_flash_queue_order(self.h, C.byref(v)), "queue order")
+        return bool(v.value)
 
     def CalFlash(self, mode=PLAIN):
         self._chk(self.lib.ocpx_flash_der(self.h, mode), "flash")

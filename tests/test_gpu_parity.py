@@ -283,10 +283,35 @@ def test_autotuner_picks_a_valid_mapping():
     wl, _ = W.flash_cells(20000, 6, seed=3)
     dev = IsoTFIM.from_workload(wl)
     ms = dev.autotune_flash()
-    assert set(ms) == {1, 4, 8} and all(v > 0 for v in ms.values())
+    nat = ms.pop("natural_order")
+    assert set(ms) == {1, 4, 8} and all(v > 0 for v in ms.values()) and nat > 0
     assert dev.flash_mapping() == min(ms, key=ms.get)
+    assert dev.flash_queue_order() == (ms[dev.flash_mapping()] <= nat)
     dev.set_flash_mapping(0)
     assert dev.flash_mapping() == min(ms, key=ms.get)
+
+
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_queue_order_does_not_change_results(lanes):
+    """The longest-first queue only reorders work: a second batched flash
+    (ordered by the first one's counts) gives the same bits as a flash in
+    natural order."""
+    wl, _ = W.flash_cells(20000, 6, seed=11, P=np.linspace(5.0, 250.0, 20000))
+    a = IsoTFIM.from_workload(wl)
+    b = IsoTFIM.from_workload(wl)
+    for d in (a, b):
+        d.set_flash_mapping(lanes)
+    a.set_flash_queue_order(True)
+    b.set_flash_queue_order(False)
+    for _ in range(2):
+        a.set_state(wl.state)
+        b.set_state(wl.state)
+        a.CalFlash(BATCHED)
+        b.CalFlash(BATCHED)
+    for f in ("PHASEEXIST", "S", "XIJ", "DSEC_DPRI", "ITERS"):
+        assert np.array_equal(a.get(f), b.get(f)), f
+    a.close()
+    b.close()
 
 
 def test_graph_replay_matches_eager():
