@@ -675,10 +675,25 @@ constexpr int QB = 4096;
 
 __device__ __forceinline__ int qbucket(int32_t e) { return QB - 1 - (e < 0 ? 0 : (e > QB - 1 ? QB - 1 : e)); }
 
+// Counts go through a block histogram in LDS first, so the global atomics
+// are one per block and bucket instead of one per cell (most cells share a
+// handful of counts).
+constexpr int QTILE = 2048;                      // cells per block
+
 __global__ void __launch_bounds__(256) qorder_hist(const int32_t* __restrict__ evals, int64_t nb, unsigned* hist)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) atomicAdd(&hist[qbucket(evals[i])], 1u);
+    __shared__ unsigned sh[QB];
+    const int t = threadIdx.x;
+    for (int b = t; b < QB; b += 256) sh[b] = 0;
+    __syncthreads();
+    const int64_t c0 = (int64_t)blockIdx.x * QTILE;
+    for (int k = t; k < QTILE; k += 256) {
+        const int64_t i = c0 + k;
+        if (i < nb) atomicAdd(&sh[qbucket(evals[i])], 1u);
+    }
+    __syncthreads();
+    for (int b = t; b < QB; b += 256)
+        if (sh[b]) atomicAdd(&hist[b], sh[b]);
 }
 
 // exclusive prefix sum of hist[QB] in place, one block of 256 threads
@@ -701,11 +716,33 @@ __global__ void __launch_bounds__(256) qorder_scan(unsigned* hist)
     UNROLL for (int k = 0; k < PER; k++) { hist[t * PER + k] = run; run += loc[k]; }
 }
 
+// each block reserves one range per bucket it holds, then ranks its cells
+// inside the range with LDS atomics
 __global__ void __launch_bounds__(256) qorder_scatter(const int32_t* __restrict__ evals, int64_t nb, unsigned* offs,
                                                       int32_t* __restrict__ order)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb) order[atomicAdd(&offs[qbucket(evals[i])], 1u)] = (int32_t)i;
+    __shared__ unsigned cnt[QB], base[QB];
+    const int t = threadIdx.x;
+    for (int b = t; b < QB; b += 256) cnt[b] = 0;
+    __syncthreads();
+    const int64_t c0 = (int64_t)blockIdx.x * QTILE;
+    for (int k = t; k < QTILE; k += 256) {
+        const int64_t i = c0 + k;
+        if (i < nb) atomicAdd(&cnt[qbucket(evals[i])], 1u);
+    }
+    __syncthreads();
+    for (int b = t; b < QB; b += 256) {
+        base[b] = cnt[b] ? atomicAdd(&offs[b], cnt[b]) : 0u;
+        cnt[b] = 0;
+    }
+    __syncthreads();
+    for (int k = t; k < QTILE; k += 256) {
+        const int64_t i = c0 + k;
+        if (i < nb) {
+            const int b = qbucket(evals[i]);
+            order[base[b] + atomicAdd(&cnt[b], 1u)] = (int32_t)i;
+        }
+    }
 }
 
 int queue_order_buckets() { return QB; }
@@ -713,7 +750,7 @@ int queue_order_buckets() { return QB; }
 hipError_t run_queue_order(const int32_t* evals, int64_t nb, unsigned* hist, int32_t* order, hipStream_t st)
 {
     if (nb == 0) return hipSuccess;
-    const unsigned g = (unsigned)((nb + 255) / 256);
+    const unsigned g = (unsigned)((nb + QTILE - 1) / QTILE);
     hipLaunchKernelGGL(qorder_hist, dim3(g), dim3(256), 0, st, evals, nb, hist);
     hipLaunchKernelGGL(qorder_scan, dim3(1), dim3(256), 0, st, hist);
     hipLaunchKernelGGL(qorder_scatter, dim3(g), dim3(256), 0, st, evals, nb, hist, order);
