@@ -192,6 +192,9 @@ int  ocpx_ctx_create_multi(int32_t nmodel, const ocpx_fluid* fluid, const ocpx_s
                            const ocpx_rock* rock, const ocpx_grid* grids, void* stream,
                            ocpx_ctx** out);
 void ocpx_ctx_destroy(ocpx_ctx* ctx);
+/* The context's last error; with ctx == NULL, why the last ocpx_ctx_create or
+ * ocpx_ctx_create_multi on this thread failed (a grid needs at least one
+ * cell). */
 const char* ocpx_last_error(const ocpx_ctx* ctx);
 int  ocpx_sync(ocpx_ctx* ctx);
 

@@ -413,20 +413,34 @@ int64_t ocpx_field_len(int field, int32_t Nc) { return flen(field, Nc); }
 namespace {
 // nmodel models share one context: fl, sat, rock are arrays of nmodel, g is
 // the concatenated grid and mid (nmodel > 1) the model of each cell.
+// Why the last context creation on this thread failed (no context exists to
+// hold the message); ocpx_last_error(NULL) returns it.
+thread_local std::string g_create_err;
+
+int create_fail(int code, const char* msg)
+{
+    g_create_err = msg;
+    return code;
+}
+
 int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* rock,
                     const ocpx_grid* g, const int32_t* mid, void* stream, ocpx_ctx** out)
 {
-    if (!out) return OCPX_E_ARG;
+    g_create_err.clear();
+    if (!out) return create_fail(OCPX_E_ARG, "null output pointer");
     *out = nullptr;
-    if (!fl || !sat || !rock || !g || nmodel < 1) return OCPX_E_ARG;
+    if (!fl || !sat || !rock || !g || nmodel < 1) return create_fail(OCPX_E_ARG, "null fluid, tables, rock or grid");
     for (int m = 0; m < nmodel; m++) {
-        if (fl[m].nc < 2 || fl[m].nc > 12) return OCPX_E_UNSUPPORTED;
-        if (fl[m].nc != fl[0].nc) return OCPX_E_ARG;
+        if (fl[m].nc < 2 || fl[m].nc > 12)
+            return create_fail(OCPX_E_UNSUPPORTED, "hydrocarbon components must be 2 to 12");
+        if (fl[m].nc != fl[0].nc) return create_fail(OCPX_E_ARG, "models of one context must share Nc");
         if (sat[m].nswof < 1 || sat[m].nswof > OCPX_MAX_TABROWS || sat[m].nsgof < 1 ||
             sat[m].nsgof > OCPX_MAX_TABROWS)
-            return OCPX_E_ARG;
+            return create_fail(OCPX_E_ARG, "SWOF/SGOF row count outside 1..OCPX_MAX_TABROWS");
     }
-    if (g->nb <= 0 || g->nb > INT32_MAX || g->nconn < 0 || g->nconn > INT32_MAX) return OCPX_E_ARG;
+    if (g->nb <= 0) return create_fail(OCPX_E_ARG, "grid with no cells");
+    if (g->nb > INT32_MAX || g->nconn < 0 || g->nconn > INT32_MAX)
+        return create_fail(OCPX_E_ARG, "cell or connection count outside int32");
     ocpx_ctx* c = new ocpx_ctx();
     c->NC = fl->nc;
     c->nc = fl->nc + 1;
@@ -626,7 +640,11 @@ void ocpx_ctx_destroy(ocpx_ctx* c)
     delete c;
 }
 
-const char* ocpx_last_error(const ocpx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* ocpx_last_error(const ocpx_ctx* c)
+{
+    if (c) return c->err.c_str();
+    return g_create_err.empty() ? "null context" : g_create_err.c_str();
+}
 
 int ocpx_sync(ocpx_ctx* c)
 {

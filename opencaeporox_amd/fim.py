@@ -41,7 +41,7 @@ class IsoTFIM:
                                       stream, C.byref(h))
         self.h = h
         if rc != 0:
-            msg = self.lib.ocpx_last_error(h).decode() if h.value else ""
+            msg = self.lib.ocpx_last_error(h if h.value else None).decode()
             if h.value:
                 self.lib.ocpx_ctx_destroy(h)
                 self.h = C.c_void_p()
@@ -79,7 +79,7 @@ class IsoTFIM:
         rc = obj.lib.ocpx_ctx_create_multi(n, fl, sa, ro, gr, stream, C.byref(h))
         obj.h = h
         if rc != 0:
-            msg = obj.lib.ocpx_last_error(h).decode() if h.value else ""
+            msg = obj.lib.ocpx_last_error(h if h.value else None).decode()
             raise OcpxError(f"ocpx_ctx_create_multi failed ({rc}): {msg}")
         for k in wls[0].state:
             obj.set(k, np.concatenate([np.asarray(w.state[k]).ravel() for w in wls]))

@@ -1,0 +1,67 @@
+"""Edge cases of the device path against the oracle: the smallest and
+largest component counts the kernels are built for, a single cell without
+connections, and a grid without cells."""
+import numpy as np
+import pytest
+
+from opencaeporox_amd import workload as W
+from opencaeporox_amd.fim import BATCHED, PLAIN, IsoTFIM
+from oracle.oracle import state_from_workload
+from tests.conftest import close, perturbed_oracle_states
+from tests.test_gpu_parity import _clean_grid, _compare_assembly, _compare_flash, _upload_flash
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("Nc,mode,lanes", [(2, PLAIN, 1), (2, BATCHED, 4), (12, BATCHED, 1), (12, BATCHED, 16)])
+def test_component_count_extremes(oracle, oracle_fma, Nc, mode, lanes):
+    wl = _clean_grid(seed=40 + Nc, Nc=Nc, nx=3, ny=2, nz=2)
+    dev = IsoTFIM.from_workload(wl)
+    dev.set_flash_mapping(lanes)
+    st = state_from_workload(wl)
+    others = perturbed_oracle_states(oracle, oracle_fma, wl.fluid, st)
+    dev.CalFlash(mode)
+    oracle.cal_flash(wl.fluid, st)
+    _compare_flash(dev, st, wl.Nc, f"Nc={Nc}", others, max_mismatch=0.1)
+    _upload_flash(dev, st)                       # assembly from identical flash outputs
+    _compare_assembly(dev, oracle, wl, st, 1.0, f"Nc={Nc}")
+    dev.close()
+
+
+def _cells(nb, Nc=6, seed=3):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    g = W.Grid(nb, 1, 1, np.full(nb, 226.5), np.full(nb, 2000.0), np.full(nb, 0.2),
+               np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    fl = W.make_fluid(Nc, 100 + seed)
+    z = rng.dirichlet(np.ones(Nc), nb) if nb else np.zeros((0, Nc))
+    st = W.initial_state(fl, W.make_rock(), g, np.full(nb, 150.0), z, rng)
+    return W.Workload(f"{nb} cells", fl, W.make_sat(), W.make_rock(), g, st)
+
+
+def test_single_cell_without_connections(oracle):
+    wl = _cells(1)
+    dev = IsoTFIM.from_workload(wl)
+    st = state_from_workload(wl)
+    dev.CalFlash(BATCHED)
+    oracle.cal_flash(wl.fluid, st)
+    for f in ("PHASEEXIST", "S", "XIJ", "DSEC_DPRI"):
+        ok, worst = close(dev.get(f), st.a[f])
+        assert ok, f"{f} worst {worst}"
+    _upload_flash(dev, st)
+    _compare_assembly(dev, oracle, wl, st, 1.0, "one cell")
+    rp, col, val, rhs = dev.bsr()
+    assert rp.tolist() == [0, 1] and col.tolist() == [0]
+    u, it, rr = dev.LinearSolve(rtol=1e-12, maxit=50)
+    assert it > 0
+    ref = np.linalg.solve(val[0].reshape(dev.bs, dev.bs), rhs)
+    assert np.allclose(u, ref, rtol=1e-10, atol=1e-12 * np.abs(ref).max())
+    dev.close()
+
+
+def test_empty_grid_is_rejected():
+    """A context needs at least one cell; creation fails with a message
+    instead of a context whose kernels have nothing to launch."""
+    from opencaeporox_amd.fim import OcpxError
+    wl = _cells(0)
+    with pytest.raises(OcpxError, match="no cells"):
+        IsoTFIM.from_workload(wl)
