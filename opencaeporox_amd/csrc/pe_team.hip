@@ -924,15 +924,18 @@ __global__ void __launch_bounds__(64) pe_team_plain(const DevFluid* __restrict__
 }
 
 // Persistent teams over the global work queue (counter io.queue, zeroed
-// before the launch).  Only team leaders vote; a team whose cell is done
-// writes it back and takes the next index.  The wavefront leaves when the
-// queue is drained and no team has a cell.
+// before the launch; io.qorder maps queue positions to cells).  Only team
+// leaders vote; a team whose cell is done writes it back and takes the next
+// index.  The wavefront leaves when the queue is drained and no team has a
+// cell; a team waiting for a Newton batch steps as soon as only Newton
+// teams are left, so every team reaches the exit.
 template <int NC, int T, bool MULTI>
 __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict__ fp, PeIO io)
 {
     const Lane<NC, T> ln = make_lane<NC, T>();
     const int lane = __lane_id();
     const int lead = lane & ~(T - 1);
+    constexpr int NR_BATCH = (64 / T) / 4;
     LaneFluid<NC, T> lf;
     Cell<NC, T> C;
     C.st = ST_DONE;
@@ -959,7 +962,19 @@ __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict
             }
         }
         if (!__any(have)) break;
-        if (have && C.st != ST_DONE) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, ln);
+        // Newton steps (matrix build + solve) cost several SSM steps: with 8
+        // or more teams in a wavefront, a team that wants one waits until a
+        // quarter of the teams do (or nothing else is left), so the Newton
+        // branch runs for several teams at once.
+        const bool live = have && C.st != ST_DONE;
+        if constexpr (NR_BATCH > 1) {
+            const bool nr = live && (C.st == ST_T_NR || C.st == ST_S_NR);
+            const unsigned long long mnr = __ballot(nr && ln.t == 0), mlive = __ballot(live && ln.t == 0);
+            const bool run_nr = __popcll(mnr) >= NR_BATCH || mnr == mlive;
+            if (live && (!nr || run_nr)) step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, ln);
+        } else if (live) {
+            step<NC, T>(fluid_of<NC, T, MULTI>(fp, C), lf, io, C, ln);
+        }
     }
 }
 
