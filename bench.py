@@ -240,7 +240,7 @@ def assembly_measure(spec, steps, warmup, sync, dt, key=None, Nc=6, solve=True):
     from opencaeporox_amd.fim import BATCHED, IsoTFIM
     from opencaeporox_amd.roofline import PEAK_HBM_GBS, assembly_bytes
     nx, ny, nz = (int(v) for v in spec.split("x"))
-    wl = W.config3(nx, ny, nz, Nc=Nc)
+    wl = W.heterogeneous_grid(nx, ny, nz, Nc=Nc)
     dev = IsoTFIM.from_workload(wl)
     dev.UpdateLastTimeStep()
     dev.autotune_flash()
@@ -292,7 +292,7 @@ def adjoint_measure(spec, steps, sync, dt):
     from opencaeporox_amd.fim import BATCHED, IsoTFIM
     from opencaeporox_amd.roofline import PEAK_HBM_GBS
     nx, ny, nz = (int(v) for v in spec.split("x"))
-    wl = W.config4(nx, ny, nz, Nc=4)
+    wl = W.channelized_grid(nx, ny, nz, Nc=4)
     dev = IsoTFIM.from_workload(wl)
     dev.UpdateLastTimeStep()
     dev.autotune_flash()
@@ -331,7 +331,7 @@ def multimodel_measure(nmodels, steps, sync, dt):
     from opencaeporox_amd import workload as W
     from opencaeporox_amd.fim import MultiModelBatch
     t = time.perf_counter()
-    wls = W.config5(nmodels)
+    wls = W.uneven_batch(nmodels)
     batch = MultiModelBatch(wls)
     log(f"multi-model batch: {len(wls)} models, {batch.nb} cells, {len(batch.groups)} Nc groups, "
         f"set up in {time.perf_counter() - t:.1f} s")

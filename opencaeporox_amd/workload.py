@@ -261,7 +261,7 @@ def initial_state(fluid, rock, grid, P, z, rng, Sw=0.2, perturb=True):
     return st
 
 
-def config1(seed=1):
+def cube_grid(seed=1):
     """10x10x10 cubes of 20 ft, ln k ~ N(ln 100, 1), phi ~ U(0.1, 0.3), Nc = 6."""
     rng = np.random.Generator(np.random.PCG64(seed))
     nx = ny = nz = 10
@@ -274,10 +274,10 @@ def config1(seed=1):
     P = rng.uniform(100.0, 280.0, nb)
     z = rng.dirichlet(np.ones(6), nb)
     st = initial_state(fl, rock, grid, P, z, rng)
-    return Workload("C1 10x10x10 Nc=6", fl, make_sat(), rock, grid, st)
+    return Workload("10x10x10 cubes, Nc=6", fl, make_sat(), rock, grid, st)
 
 
-def config3(nx=100, ny=100, nz=50, Nc=6, seed=3):
+def heterogeneous_grid(nx=100, ny=100, nz=50, Nc=6, seed=3):
     """Hydrostatic P + N(0, 1 MPa), ln kx ~ N(ln 100, 2), kz = 0.1 kx."""
     rng = np.random.Generator(np.random.PCG64(seed))
     nb = nx * ny * nz
@@ -289,7 +289,7 @@ def config3(nx=100, ny=100, nz=50, Nc=6, seed=3):
     P = 200.0 + 0.0980665 * (grid.depth - grid.depth.min()) + rng.normal(0, 10.0, nb)
     z = rng.dirichlet(np.ones(Nc), nb)
     st = initial_state(fl, rock, grid, P, z, rng)
-    return Workload(f"C3 {nx}x{ny}x{nz} Nc={Nc}", fl, make_sat(), rock, grid, st)
+    return Workload(f"{nx}x{ny}x{nz} heterogeneous grid, Nc={Nc}", fl, make_sat(), rock, grid, st)
 
 
 def channel_facies(nx, ny, nz, rng, nchan=6, width=0.035):
@@ -311,7 +311,7 @@ def channel_facies(nx, ny, nz, rng, nchan=6, width=0.035):
     return fac.ravel()
 
 
-def config4(nx=60, ny=220, nz=85, Nc=4, seed=4, lbc_exact=0):
+def channelized_grid(nx=60, ny=220, nz=85, Nc=4, seed=4, lbc_exact=0):
     """Channelized facies (2000 / 1 mD means, log-normal sigma = 1 inside
     each facies), Nc = 4, hydrostatic P + N(0, 1 MPa)."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -326,7 +326,7 @@ def config4(nx=60, ny=220, nz=85, Nc=4, seed=4, lbc_exact=0):
     P = 200.0 + 0.0980665 * (grid.depth - grid.depth.min()) + rng.normal(0, 10.0, nb)
     z = rng.dirichlet(np.ones(Nc), nb)
     st = initial_state(fl, rock, grid, P, z, rng)
-    return Workload(f"C4 {nx}x{ny}x{nz} Nc={Nc} channelized", fl, make_sat(), rock, grid, st)
+    return Workload(f"{nx}x{ny}x{nz} channelized grid, Nc={Nc}", fl, make_sat(), rock, grid, st)
 
 
 def model_box(ncell, nz=10):
@@ -335,7 +335,7 @@ def model_box(ncell, nz=10):
     return side, side, nz
 
 
-def config5(nmodels=256, seed=5, cmin=5000, cmax=200000):
+def uneven_batch(nmodels=256, seed=5, cmin=5000, cmax=200000):
     """Uneven multi-model batch: cell counts log-uniform in [cmin, cmax],
     Nc uniform in {3..12}, each model with its own seeded PR fluid,
     log-normal permeability, porosity and pressures."""
@@ -394,7 +394,7 @@ def saturation_pressure(nphase, z, T, lo=5.0, hi=700.0, coarse=24, iters=30):
     return np.where(ok, 0.5 * (a + b), np.nan)
 
 
-def config2_pressures(n, rng, psat, frac_near=0.3, band=0.05):
+def near_saturation_pressures(n, rng, psat, frac_near=0.3, band=0.05):
     """30% of the cells within +-5% of their saturation pressure, the rest
     (and cells with no saturation pressure) U(10, 28) MPa."""
     P = rng.uniform(100.0, 280.0, n)

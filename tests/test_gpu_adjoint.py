@@ -20,7 +20,7 @@ def _problem(nx=12, ny=44, nz=17):
     # exact LBC derivative: the reference's d(mu)/dx drops a term (integer
     # division, src/OCPFuncPVT.cpp:537), so its Jacobian is not the exact
     # tangent; the gradient check needs the exact one
-    wl = W.config4(nx, ny, nz, Nc=4, seed=4, lbc_exact=1)
+    wl = W.channelized_grid(nx, ny, nz, Nc=4, seed=4, lbc_exact=1)
     dev = IsoTFIM.from_workload(wl)
     prob = FIMProblem(dev, dt=1.0, lNi=wl.state["LNI"])
     return wl, dev, prob

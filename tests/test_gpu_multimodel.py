@@ -54,7 +54,7 @@ def test_merged_context_equals_separate_contexts(lanes):
 
 
 def test_multimodel_batch_groups_by_component_count():
-    wls = W.config5(nmodels=6, seed=3, cmin=200, cmax=2000)
+    wls = W.uneven_batch(nmodels=6, seed=3, cmin=200, cmax=2000)
     batch = MultiModelBatch(wls)
     assert sorted(batch.groups) == sorted({w.fluid.nc for w in wls})
     batch.autotune()

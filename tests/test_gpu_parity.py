@@ -67,14 +67,14 @@ def _compare_flash(dev, st, Nc, tag, others, max_mismatch=0.02):
 
 @pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 4), (BATCHED, 8)])
 def test_flash_config1(oracle, oracle_fma, mode, lanes):
-    wl = W.config1()
+    wl = W.cube_grid()
     dev = IsoTFIM.from_workload(wl)
     dev.set_flash_mapping(lanes)
     st = state_from_workload(wl)
     others = perturbed_oracle_states(oracle, oracle_fma, wl.fluid, st)
     dev.CalFlash(mode)
     oracle.cal_flash(wl.fluid, st)
-    same = _compare_flash(dev, st, wl.Nc, f"C1 mode {mode} lanes {lanes}", others)
+    same = _compare_flash(dev, st, wl.Nc, f"10x10x10 mode {mode} lanes {lanes}", others)
     # SkipPSA next-step data
     keep = same & ~rounding_sensitive(st.ill)
     assert np.array_equal(dev.get("SKIP_FLAG")[keep], st.a["SKIP_FLAG"][keep])
@@ -195,12 +195,12 @@ def test_assembly_config1_same_flash(oracle):
     """Residual and Jacobian from identical flash outputs: the oracle's flash
     state is uploaded, then kr/Pc, rock, residual and assembly run on the
     device."""
-    wl = W.config1()
+    wl = W.cube_grid()
     st = state_from_workload(wl)
     oracle.cal_flash(wl.fluid, st)
     dev = IsoTFIM.from_workload(wl)
     _upload_flash(dev, st)
-    _compare_assembly(dev, oracle, wl, st, 1.0, "C1")
+    _compare_assembly(dev, oracle, wl, st, 1.0, "10x10x10")
 
 
 def _clean_grid(seed=31, Nc=3, nx=4, ny=3, nz=2):
@@ -248,7 +248,7 @@ def test_newton_passes_end_to_end(oracle, oracle_fma, mode, lanes):
 
 
 def test_reset_to_last_step_repeats_the_pass():
-    wl = W.config1()
+    wl = W.cube_grid()
     dev = IsoTFIM.from_workload(wl)
     dev.UpdateLastTimeStep()
     dev.newton_pass(1.0, BATCHED)
@@ -263,7 +263,7 @@ def test_reset_to_last_step_repeats_the_pass():
 
 
 def test_jt_lambda_dot_product():
-    wl = W.config1()
+    wl = W.cube_grid()
     dev = IsoTFIM.from_workload(wl)
     dev.newton_pass(1.0, BATCHED)
     rp, col, val, _ = dev.bsr()
@@ -315,7 +315,7 @@ def test_queue_order_does_not_change_results(lanes):
 
 
 def test_graph_replay_matches_eager():
-    wl = W.config1()
+    wl = W.cube_grid()
     a = IsoTFIM.from_workload(wl)
     b = IsoTFIM.from_workload(wl)
     a.newton_pass(1.0, BATCHED, graph=False)

@@ -20,17 +20,17 @@ def _system(dev):
 
 
 def test_linear_solve_matches_direct_solve():
-    wl = W.config1()
+    wl = W.cube_grid()
     dev = IsoTFIM.from_workload(wl)
     dev.newton_pass(1.0, BATCHED)
     J, rhs = _system(dev)
     u, it, rr = dev.LinearSolve(rtol=1e-12, maxit=2000)
-    print(f"C1: {it} iterations, ||r||/||b|| {rr:.2e}")
+    print(f"10x10x10: {it} iterations, ||r||/||b|| {rr:.2e}")
     assert it > 0 and rr < 1e-12
     true = np.linalg.norm(rhs - J @ u) / np.linalg.norm(rhs)
     ref = spla.spsolve(J.tocsc(), rhs)
     err = np.linalg.norm(u - ref) / np.linalg.norm(ref)
-    print(f"C1: true residual {true:.2e}, |u - u_direct| / |u_direct| {err:.2e}")
+    print(f"10x10x10: true residual {true:.2e}, |u - u_direct| / |u_direct| {err:.2e}")
     assert true < 1e-11
     assert err < 1e-8
     dev.close()
@@ -38,7 +38,7 @@ def test_linear_solve_matches_direct_solve():
 
 def test_linear_solve_reference_tolerance_on_heterogeneous_grid():
     # the reference's solver settings (1e-3 on ||r|| / ||b||, 200 iterations)
-    wl = W.config3(20, 20, 10, Nc=6)
+    wl = W.heterogeneous_grid(20, 20, 10, Nc=6)
     dev = IsoTFIM.from_workload(wl)
     dev.newton_pass(1.0, BATCHED)
     J, rhs = _system(dev)
@@ -51,7 +51,7 @@ def test_linear_solve_reference_tolerance_on_heterogeneous_grid():
 
 
 def test_nonconvergence_is_reported():
-    wl = W.config3(20, 20, 10, Nc=6)
+    wl = W.heterogeneous_grid(20, 20, 10, Nc=6)
     dev = IsoTFIM.from_workload(wl)
     dev.newton_pass(1.0, BATCHED)
     u, it, rr = dev.LinearSolve(rtol=1e-30, maxit=3)
