@@ -1,0 +1,50 @@
+"""Measurement tooling on the CPU: the PMC traffic summary and the bench's
+lookup of a committed traffic measurement."""
+import csv
+import json
+
+import pytest
+
+from opencaeporox_amd import pmcsum
+
+
+def _csv(path, rows, counter):
+    cols = ["Correlation_Id", "Dispatch_Id", "Kernel_Name", "Counter_Name", "Counter_Value"]
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=cols)
+        w.writeheader()
+        for d, (name, v) in enumerate(rows, 1):
+            w.writerow({"Correlation_Id": d, "Dispatch_Id": d, "Kernel_Name": name, "Counter_Name": counter,
+                        "Counter_Value": v})
+
+
+def test_pmcsum_calibrates_against_the_copy(tmp_path):
+    # a 1 GiB copy whose FETCH_SIZE reads half its bytes and WRITE_SIZE all of them
+    _csv(tmp_path / "cf.csv", [("__amd_rocclr_copyBuffer", 524288.0)], "FETCH_SIZE")
+    _csv(tmp_path / "cw.csv", [("__amd_rocclr_copyBuffer", 1048576.0)], "WRITE_SIZE")
+    # a kernel dispatched twice; --last 1 keeps the second dispatch
+    _csv(tmp_path / "f.csv", [("k<1>", 10.0), ("k<1>", 100.0)], "FETCH_SIZE")
+    _csv(tmp_path / "w.csv", [("k<1>", 20.0), ("k<1>", 50.0)], "WRITE_SIZE")
+    out = tmp_path / "t.json"
+    pmcsum.main(["--fetch", str(tmp_path / "f.csv"), "--write", str(tmp_path / "w.csv"),
+                 "--copy-fetch", str(tmp_path / "cf.csv"), "--copy-write", str(tmp_path / "cw.csv"),
+                 "--copy-mib", "1024", "--last", "1", "--workload", "key", "--json", str(out)])
+    d = json.loads(out.read_text())
+    assert d["fetch_factor"] == pytest.approx(2.0) and d["write_factor"] == pytest.approx(1.0)
+    assert d["workload"] == "key"
+    k = d["kernels"]["k<1>"]
+    assert k["dispatches"] == 1 and k["fetch_kib"] == 100.0 and k["write_kib"] == 50.0
+    assert k["hbm_bytes"] == pytest.approx((100.0 * 2 + 50.0) * 1024)
+
+
+def test_bench_reports_traffic_only_for_its_own_configuration(tmp_path, monkeypatch):
+    import bench
+    f = tmp_path / "traffic.json"
+    f.write_text(json.dumps({"workload": "cells=8 Nc=9 seed=2 assembly=4x4x4",
+                             "kernels": {"void ocpx::res_asm<7>(...)": {"hbm_bytes": 123.0}}}))
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", str(f))
+    assert bench.measured_traffic("res_asm<7>", "cells=8 Nc=9 seed=2 assembly=4x4x4") == 123.0
+    assert bench.measured_traffic("res_asm<7>", "cells=16 Nc=9 seed=2 assembly=4x4x4") is None
+    assert bench.measured_traffic("res_asm<6>", "cells=8 Nc=9 seed=2 assembly=4x4x4") is None
+    monkeypatch.setattr(bench, "TRAFFIC_FILE", str(tmp_path / "missing.json"))
+    assert bench.measured_traffic("res_asm<7>", "cells=8 Nc=9 seed=2 assembly=4x4x4") is None
