@@ -228,17 +228,18 @@ class Workload:
         return self.fluid.nc
 
 
-def initial_state(fluid, rock, grid, P, z, rng, Sw=0.2, perturb=True):
+def initial_state(fluid, rock, grid, P, z, rng, Sw=0.2, perturb=True, T=344.0):
     """BulkVarSet-layout arrays for the first Newton flash of a time step:
     water-only previous phase state, Ni filling the pore volume at P
     (InitFlashDer semantics with the single-phase molar volume,
     src/OCPMixtureMethodComp.cpp:732-743, 821-837), lNi = Ni and then a
-    seeded +-1% Ni and +-0.1 MPa P perturbation so residuals are non-zero."""
+    seeded +-1% Ni and +-0.1 MPa P perturbation so residuals are non-zero.
+    T is the temperature in K (scalar or per cell)."""
     fa = fluid_arrays(fluid)
     Nc = fluid.nc
     nc, np_ = Nc + 1, 3
     nb = grid.nb
-    T = np.full(nb, 344.0)
+    T = np.broadcast_to(np.asarray(T, float), (nb,)).copy()
     dP = P - rock.Pref
     Vp = grid.v * grid.poro * (1 + (rock.cp1 + rock.cp2 / 2 * dP) * dP)
     vm = pr_single_phase_vm(fa, P, T, z)

@@ -261,6 +261,7 @@ class Ref:
         L.ref_lbc.argtypes = [vp, dbl, vp, dbl, dbl, vp, vp, vp]
         L.ref_kr_pc.argtypes = [vp, vp, vp, vp, vp, vp]
         L.ref_rock.argtypes = [vp, dbl, dbl, vp]
+        L.ref_eos_deck.argtypes = [C.c_char_p, C.c_int] + [vp] * 6 + [C.c_int, dbl, dbl] + [vp] * 4
 
     @staticmethod
     def available():
@@ -313,3 +314,12 @@ class Ref:
 
     def rock(self, rk, P, poroInit):
         o = np.zeros(2); self.lib.ref_rock(C.byref(rk), P, poroInit, _p(o)); return o
+
+    def eos_deck(self, units, Tc, Pc, acf, OmegaA, OmegaB, bic, P, T, x):
+        """phi, (A, B, Z), Z*R*T/P of a fluid given in a deck's own units."""
+        a = [None if v is None else np.ascontiguousarray(v, D) for v in (Tc, Pc, acf, OmegaA, OmegaB, bic, x)]
+        n = len(a[0])
+        phi, abz, vm = np.zeros(n), np.zeros(3), np.zeros(1)
+        self.lib.ref_eos_deck(units.encode(), n, *[_p(v) for v in a[:6]], len(a[5]), P, T, _p(a[6]),
+                              _p(phi), _p(abz), _p(vm))
+        return phi, abz, vm[0]

@@ -23,10 +23,10 @@ namespace {
 
 // SetUnit writes globals defined in src/OCPUnits.cpp (std::string among
 // them), so it must run after static initialisation: call it lazily.
+bool units_set = false;
 void metric()
 {
-    static bool done = false;
-    if (!done) { SetUnit("METRIC"); done = true; }
+    if (!units_set) { SetUnit("METRIC"); units_set = true; }
 }
 
 vector<double> take(const double* a, int n) { return vector<double>(a, a + n); }
@@ -127,6 +127,42 @@ double ref_eos_vmder(const ocpx_fluid* f, double P, double T, const double* x, d
     double v = e.CalVmDer(P, T, x, vp, vmx);
     *vmP = vp;
     return v;
+}
+
+// A fluid as a deck gives it, in the deck's own unit system ("FIELD" or
+// "METRIC"), through the reference's EoS set-up (src/OCPEoS.cpp:20-59: the
+// Omega defaults, the flat lower-triangle BIC expanded to the symmetric
+// matrix).  OmegaA / OmegaB may be null (defaults).  Returns phi, (A, B, Z)
+// and the molar volume Z*R*T/P in that unit system; the units are METRIC
+// again afterwards.
+void ref_eos_deck(const char* units, int nc, const double* Tc, const double* Pc, const double* acf,
+                  const double* OmegaA, const double* OmegaB, const double* bic, int nbic,
+                  double P, double T, const double* x, double* phi, double* ABZ, double* vm)
+{
+    SetUnit(units);
+    units_set = true;
+    ComponentParam p;
+    p.numCom = nc;
+    p.numPhase = 2;
+    auto set = [&](Type_A_r<vector<double>>& t, const double* a) {
+        if (!a) return;
+        t.activity = 1;
+        t.data.assign(1, take(a, nc));
+    };
+    set(p.Tc, Tc);
+    set(p.Pc, Pc);
+    set(p.Acf, acf);
+    set(p.OmegaA, OmegaA);
+    set(p.OmegaB, OmegaB);
+    p.BIC.assign(1, take(bic, nbic));
+    {
+        EoSProbe e(p);
+        vector<double> fug(nc);
+        e.CalFugPhi(P, T, x, fug.data(), phi);
+        ABZ[0] = e.A(); ABZ[1] = e.B(); ABZ[2] = e.Z();
+        *vm = e.Z() * GAS_CONSTANT * T / P;
+    }
+    SetUnit("METRIC");
 }
 
 void ref_pvtw(const ocpx_fluid* f, double P, double* o)

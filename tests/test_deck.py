@@ -1,0 +1,262 @@
+"""Deck reader (opencaeporox_amd/deck.py) on the CPU.
+
+The SPE3 and SPE5 decks under tests/golden/decks are the reference's own
+input files (data/spe3, data/spe5).  The tokeniser is checked case by case
+against the rules of src/UtilInput.cpp:14-76; the FIELD -> METRIC
+conversion is pinned three ways:
+  - the reference's own EoS set-up run in FIELD units on the raw deck values
+    (oracle/_ref, this container only) gives the same ln phi, A, B, Z as the
+    oracle on the converted fluid;
+  - the reference's rock and water formulas give the same porosity, Bw and
+    viscosity from the raw and the converted values;
+  - the SPE5 oil's bubble point at 160 degF, from the oracle's flash on the
+    converted fluid, is the published 2302.3 psia (Killough & Kossack, SPE
+    16000, 1987).
+"""
+import copy
+import os
+
+import numpy as np
+import pytest
+
+from opencaeporox_amd import abi, deck
+from opencaeporox_amd import workload as W
+
+DECKS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks")
+SPE5 = os.path.join(DECKS, "spe5", "spe5.data")
+SPE3 = os.path.join(DECKS, "spe3", "spe3.data")
+
+
+@pytest.fixture(scope="module")
+def spe5():
+    return deck.read_deck(SPE5)
+
+
+@pytest.fixture(scope="module")
+def spe3():
+    return deck.read_deck(SPE3)
+
+
+# ----------------------------------------------------------------- tokeniser
+@pytest.mark.parametrize("line,expect", [
+    ("", None),
+    ("   \t ", None),
+    ("-- a comment", None),
+    ("   -- indented comment", None),
+    ("# hash comment", None),
+    ("\t# indented hash", None),
+    ("- 1 2", ["-", "1", "2"]),                      # one dash is data
+    ("1 2 -- not a comment here", ["1", "2", "--", "not", "a", "comment", "here"]),
+    ("'DX'  500  6*  /", ["DX", "500", "6*", "/"]),
+    ("1 JAN 1990 /   trailing text / more", ["1", "JAN", "1990", "/"]),
+    ("1,2,,3", ["1", "2", "3"]),
+    ("/", ["/"]),
+    ("a/b", ["a/"]),                                 # '/' glued to the token before it
+])
+def test_tokens_follow_readline(line, expect):
+    assert deck.tokens(line) == expect
+
+
+@pytest.mark.parametrize("vbuf,expect", [
+    (["3*"], ["DEFAULT"] * 3),
+    (["2*0.5", "x"], ["0.5", "0.5", "x"]),
+    (["*"], ["*"]),                                  # no count: kept
+    (["0*5"], ["0*5"]),                              # zero count: kept
+    (["1*", "/"], ["DEFAULT", "/"]),
+    (["abc"], ["abc"]),
+])
+def test_deal_default(vbuf, expect):
+    assert deck.deal_default(vbuf) == expect
+
+
+# -------------------------------------------------------------- what is read
+def test_spe5_values(spe5):
+    d = spe5
+    assert d.units == "FIELD" and d.num_com == 6
+    assert d.cnames == ["C1", "C3", "C6", "C10", "C15", "C20"]
+    assert d.comp["TCRIT"][0] == [343.0, 665.7, 913.4, 1111.8, 1270.0, 1380.0]
+    assert d.comp["ZCRIT"][0][-1] == 0.235 and d.comp["MW"][0][3] == 149.29
+    assert len(d.bic[0]) == 15
+    assert d.flash == {"RR": ["30", "1e-12"], "SSMSTA": ["100", "1e-12", "1e-8"], "NRSTA": ["55", "1e-12"],
+                       "SSMSP": ["100", "1E-6"], "NRSP": ["55", "1e-12"]}
+    assert d.rtemp == 160.0 and d.dimens == (7, 7, 3)
+    assert d.rock == [{"type": "LINEAR01", "Pref": 3990.3, "cp1": 5e-6, "cp2": 0.0}]
+    assert d.gravity_active and not d.density_active
+    sw = d.tables["SWOF"][0]
+    assert len(sw) == 4 and len(sw[0]) == 11 and sw[1][1] == 0.0022
+    assert d.tables["PVTW"][0] == [[14.7], [1.0], [3.3e-6], [0.7], [0.0]]
+    assert d.tables["ZMFVD"][0] == [[1000.0], [0.5], [0.03], [0.07], [0.2], [0.15], [0.05]]
+    assert d.equil[0][:2] == [8400.0, 4000.0]
+    assert d.welspecs[0][:4] == ["PROD1", "G", "7", "7"]
+
+
+def test_spe3_values(spe3):
+    d = spe3
+    assert d.num_com == 9 and d.cnames[0] == "CO2" and d.cnames[-1] == "C7+3"
+    assert "ACF" in d.comp and d.comp["ACF"][0][8] == 0.91692
+    assert d.comp["OMEGAA"][0][2] == 0.534021 and d.comp["OMEGAB"][0][6] == 0.0872878
+    assert len(d.bic[0]) == 36 and d.bic[0][0] == -0.02
+    assert len(d.tables["SWOF"][0][0]) == 34 and len(d.tables["SGOF"][0][0]) == 33
+    assert d.tables["PVTW"][0] == [[3550.0], [1.0], [3e-6], [0.31], [0.0]]
+    # VERSION, NCNP and ZI are not reference keywords: skipped with their data lines
+    assert {"VERSION", "NCNP", "ZI"} <= set(d.skipped)
+
+
+def test_bic_expands_lower_triangle(spe5, spe3):
+    f = spe5.to_fluid()
+    k = np.array(f.BIC[:]).reshape(abi.MAX_NC, abi.MAX_NC)[:6, :6]
+    assert np.array_equal(k, k.T) and np.all(np.diag(k) == 0)
+    # rows 4 and 5 of the deck's triangle: C15 and C20 against C1 and C3
+    assert np.all(k[3, :3] == 0) and k[4, 0] == 0.05 and k[4, 1] == 0.005
+    assert k[5, 0] == 0.05 and k[5, 1] == 0.005 and k[5, 4] == 0.0
+    f = spe3.to_fluid()
+    k = np.array(f.BIC[:]).reshape(abi.MAX_NC, abi.MAX_NC)[:9, :9]
+    assert k[1, 0] == -0.02 and k[2, 1] == 0.036 and k[8, 2] == 0.130663 and k[0, 8] == 0.1
+
+
+def test_spe5_grid(spe5):
+    """Layer-centre depths by CalDepthVOrthogonalGrid (TOPS 8325 ft, DZ 20,
+    30, 50 ft), volumes, and the two transmissibility kinds in closed form."""
+    g = spe5.to_grid()
+    ft = deck.FT
+    assert g.nb == 147 and g.nconn == 2 * 6 * 7 * 3 + 7 * 7 * 2
+    for k, dft in enumerate((8335.0, 8360.0, 8400.0)):
+        assert np.allclose(g.depth[k * 49:(k + 1) * 49], dft * ft, rtol=1e-15)
+    assert np.isclose(g.v[0], 500 * 500 * 20 * ft ** 3, rtol=1e-14)
+    # x-connection of two equal layer-1 cells: k * dy * dz / dx
+    c = np.where((g.conn_b == 0) & (g.conn_e == 1))[0][0]
+    assert np.isclose(g.trans[c], W.DARCY * 500 * (500 * 20 / 500) * ft, rtol=1e-14)
+    # z-connection layer 1 -> 2: harmonic mean of kz * 2 dx dy / dz
+    c = np.where((g.conn_b == 0) & (g.conn_e == 49))[0][0]
+    t1, t2 = 50 * 2 * 500 * 500 / 20 * ft, 50 * 2 * 500 * 500 / 30 * ft
+    assert np.isclose(g.trans[c], W.DARCY / (1 / t1 + 1 / t2), rtol=1e-14)
+    assert np.all(g.poro == 0.3)
+
+
+def test_spe3_grid_is_the_reference_connection_order(spe3):
+    g = spe3.to_grid()
+    nx, ny, nz = spe3.dimens
+    n = g.nb
+    kx, ky, kz = (spe3.arrays[k][:n] for k in ("PERMX", "PERMY", "PERMZ"))
+    dx, dy, dz = (spe3.arrays[k][:n] * deck.FT for k in ("DX", "DY", "DZ"))
+    assert g.nconn == (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
+    assert np.all(np.diff(g.conn_b) >= 0) and np.all(g.conn_e > g.conn_b)
+    # every connection's transmissibility from its own two cells (no NTG in the deck)
+    b, e = g.conn_b, g.conn_e
+    d = e - b
+    assert set(np.unique(d)) == {1, nx, nx * ny}
+
+    def half(c):
+        return np.where(d == 1, kx[c] * 2 * dy[c] * dz[c] / dx[c],
+                        np.where(d == nx, ky[c] * 2 * dz[c] * dx[c] / dy[c], kz[c] * 2 * dx[c] * dy[c] / dz[c]))
+
+    assert np.allclose(g.trans, W.DARCY / (1 / half(b) + 1 / half(e)), rtol=1e-14)
+
+
+# ------------------------------------------------------------ unit pinning
+@pytest.mark.parametrize("path", [SPE5, SPE3])
+def test_field_conversion_matches_reference_eos(path, oracle, ref):
+    d = deck.read_deck(path)
+    fl = d.to_fluid()
+    g = lambda k: None if k not in d.comp else np.array(d.comp[k][0])
+    TR, TK = d.rtemp + 459.67, d.temperature_c() + W.CONV4
+    rng = np.random.Generator(np.random.PCG64(5))
+    for P_psia in (300.0, 2000.0, 3550.0, 7000.0):
+        for _ in range(4):
+            x = rng.dirichlet(np.ones(d.num_com))
+            phi_r, abz_r, vm_r = ref.eos_deck("FIELD", g("TCRIT"), g("PCRIT"), g("ACF"), g("OMEGAA"),
+                                              g("OMEGAB"), np.array(d.bic[0]), P_psia, TR, x)
+            _, phi_o, abz_o = oracle.fug(fl, P_psia * deck.PSI, TK, x)
+            assert np.abs(np.log(phi_o) - np.log(phi_r)).max() < 1e-12
+            assert np.allclose(abz_o, abz_r, rtol=1e-13)
+            vm_o = oracle.vmder(fl, P_psia * deck.PSI, TK, x)[0]
+            # the FIELD and METRIC gas constants are 2.1e-5 apart after conversion
+            assert abs(vm_o / (vm_r * deck.FT3_PER_LBMOL) - 1) < 3e-5
+
+
+def test_rock_and_water_conversion(spe5, ref):
+    rk_si = spe5.to_rock()
+    rk_field = abi.Rock()
+    r = spe5.rock[0]
+    rk_field.Pref, rk_field.cp1, rk_field.cp2 = r["Pref"], r["cp1"], r["cp2"]
+    fl = spe5.to_fluid()
+    fl_field = copy.deepcopy(fl)
+    for k, v in enumerate(c[0] for c in spe5.tables["PVTW"][0]):
+        fl_field.PVTW[k] = v
+    fl_field.rhoW_std = 62.3664
+    for P_psia in (1000.0, 3990.3, 6000.0):
+        a = ref.rock(rk_si, P_psia * deck.PSI, 0.3)
+        b = ref.rock(rk_field, P_psia, 0.3)
+        assert np.isclose(a[0], b[0], rtol=1e-14) and np.isclose(a[1], b[1] / deck.PSI, rtol=1e-12)
+        w_si, w_f = ref.pvtw(fl, P_psia * deck.PSI), ref.pvtw(fl_field, P_psia)
+        # rho, xi, mu: rho scales with the standard density only, mu not at all
+        assert np.isclose(w_si[0] / w_f[0], 999.014 / 62.3664, rtol=1e-13)
+        assert np.isclose(w_si[2], w_f[2], rtol=1e-13)
+
+
+def test_spe5_bubble_point(spe5, oracle):
+    fl = spe5.to_fluid()
+    z = spe5.composition(np.array([8400 * deck.FT]))
+    T = np.array([spe5.temperature_c() + W.CONV4])
+
+    def nphase(P, zz, TT):
+        return oracle.phase_equilibrium(fl, P, TT, zz)[0]
+
+    pb = W.saturation_pressure(nphase, z, T, lo=50.0, hi=300.0, coarse=100, iters=40)[0] / deck.PSI
+    assert abs(pb - 2302.3) < 0.001 * 2302.3, pb
+
+
+def test_temperatures_and_units(spe5):
+    assert np.isclose(spe5.temperature_c() + W.CONV4, (160 + 459.67) * 5 / 9, rtol=1e-15)
+    f = spe5.to_fluid()
+    assert np.isclose(f.Tc[0], 343.0 * 5 / 9) and np.isclose(f.Pc[0], 667.8 * deck.PSI)
+    # Vc from ZCRIT as the reference computes it in FIELD units, then in m3/kmol
+    assert np.isclose(f.Vc[0], deck.FIELD_GAS_CONSTANT * 0.29 * 343.0 / 667.8 * deck.FT3_PER_LBMOL, rtol=1e-15)
+    assert f.rhoW_std == deck.METRIC_RHOW_STD and f.RR_maxIt == 30 and f.SSMsta_eYt == 1e-8
+
+
+# ------------------------------------------------------------------- errors
+def _write(tmp_path, text, name="d.data"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_missing_flash_controls(tmp_path):
+    d = deck.read_deck(_write(tmp_path, "COMPS\n2\n/\nTCRIT\n300 400 /\nPCRIT\n600 500 /\nZCRIT\n.29 .28 /\n"
+                                        "MW\n16 44 /\nACF\n.01 .15 /\nPVTW\n14.7 1 3e-6 .7 0 /\n/\n"))
+    with pytest.raises(deck.DeckError, match="RR hasn't been input"):
+        d.to_fluid()
+
+
+def test_component_keyword_before_comps(tmp_path):
+    with pytest.raises(deck.DeckError, match="number of components"):
+        deck.read_deck(_write(tmp_path, "TCRIT\n300 400 /\n"))
+
+
+def test_equals_outside_the_grid(tmp_path):
+    with pytest.raises(deck.DeckError, match="outside the grid"):
+        deck.read_deck(_write(tmp_path, "DIMENS\n2 2 1 /\nEQUALS\n'DX' 10 1 3 1 2 1 1 /\n/\n"))
+
+
+def test_include_and_metric_deck(tmp_path):
+    """A METRIC deck with the fluid in an INCLUDE file, VCRIT instead of
+    ZCRIT, MULTIPLY, and arrays given cell by cell with repeat counts."""
+    _write(tmp_path, "CNAMES\nA B\n/\nTCRIT\n300 400 /\nPCRIT\n40 30 /\nVCRIT\n0.1 0.2 /\n"
+                     "MW\n16 44 /\nACF\n.01 .15 /\nBIC\n0.02 /\nRR\n30 1e-12\n/\nSSMSTA\n100 1e-12 1e-8\n/\n"
+                     "NRSTA\n55 1e-12\n/\nSSMSP\n100 1e-6\n/\nNRSP\n55 1e-12\n/\n", "fl.in")
+    p = _write(tmp_path, "METRIC\nCOMPS\n2\n/\nINCLUDE\nfl.in\n/\nDIMENS\n2 1 2 /\n"
+                         "DX\n4*10\n/\nDY\n4*5\n/\nDZ\n2*1 2*2\n/\nTOPS\n2*100\n/\nPORO\n4*0.2\n/\n"
+                         "PERMX\n100 200 300 400\n/\nMULTIPLY\n'PERMX' 2 1 1 1 1 1 1 /\n/\n"
+                         "COPY\nPERMX PERMZ /\n/\nRTEMP\n80\n/\nPVTW\n1.0 1.0 4e-5 0.5 0\n/\n")
+    d = deck.read_deck(p)
+    assert d.cnames == ["A", "B"] and d.units == "METRIC"
+    f = d.to_fluid()
+    assert f.Tc[1] == 400 and f.Pc[1] == 30 and f.Vc[1] == 0.2 and f.Vcvis[1] == 0.2
+    assert f.BIC[1 * abi.MAX_NC + 0] == 0.02 and f.BIC[0 * abi.MAX_NC + 1] == 0.02
+    assert f.OmegaA[0] == deck.OMEGA_A and f.PVTW[2] == 4e-5
+    assert d.temperature_c() == 80.0
+    assert d.arrays["PERMX"].tolist() == [200, 200, 300, 400]
+    assert d.arrays["PERMZ"].tolist() == [200, 200, 300, 400]
+    g = d.to_grid()
+    assert np.allclose(g.depth, [100.5, 100.5, 102.0, 102.0])
