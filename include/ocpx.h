@@ -115,6 +115,29 @@ typedef struct ocpx_grid {
     const double*  trans;            /* [nconn]                          */
 } ocpx_grid;
 
+/*
+ * Peaceman wells (PeacemanWell / PeacemanWellIsoT, src/WellPeaceman.cpp).
+ * One well unknown (the bottom-hole pressure) per open well, as the
+ * reference's FIM well rows.  Perforations of well w are entries
+ * [perf0, perf0 + nperf) of the perforation arrays of ocpx_set_wells.
+ * The per-time-step quantities the reference computes with extra flashes
+ * are inputs here: the perforation heads dG (CaldG), the injected fluid's
+ * molar density at each perforation (XiPhase in CalFlux) and, in rate
+ * mode, the component weights of the rate equation (CalFactor).
+ */
+enum ocpx_well_type { OCPX_WELL_INJ = 0, OCPX_WELL_PROD = 1 };
+enum ocpx_well_mode { OCPX_WELL_BHP = 0, OCPX_WELL_RATE = 1 };
+
+typedef struct ocpx_well {
+    int32_t type;                    /* ocpx_well_type                   */
+    int32_t mode;                    /* ocpx_well_mode                   */
+    int32_t perf0, nperf;            /* perforation range                */
+    double  bhp;                     /* bottom-hole pressure, bar        */
+    double  target;                  /* tarBHP (bar) or tarRate          */
+    double  zi[OCPX_MAX_NC + 1];     /* injected composition (nc = Nc+1) */
+    double  factor[OCPX_MAX_NC + 1]; /* rate weight per component        */
+} ocpx_well;
+
 /* Residual norms of IsoT_FIM::CalRes (src/IsoThermalMethod.cpp:1145-1168) */
 typedef struct ocpx_resnorms {
     double maxRelRes_V;   int64_t maxId_V;
@@ -271,6 +294,49 @@ int  ocpx_bsr_device_ptrs(ocpx_ctx* ctx, void** rowptr, void** colind,
  * [nb*(nc+1)], or NULL for the device solution of the last
  * ocpx_linear_solve (no copy). */
 int  ocpx_get_solution(ocpx_ctx* ctx, const double* u, double dSmax);
+
+/* ---- Peaceman wells ------------------------------------------------------
+ * ocpx_set_wells: the wells and their perforations (perf_cell: cell index,
+ * perf_WI: WI * multiplier as PeacemanWell::CalWI leaves it, CONV_DARCY
+ * included; perf_dG: perforation head of CaldG, bar; perf_xi: molar density
+ * of the injected fluid at the perforation, injectors only, may be NULL
+ * without injectors).  A well perforates a cell at most once.  Replaces any
+ * earlier set; nwell = 0 removes the wells.  Copies the arrays.
+ * (PeacemanWell::InputPerfo / Setup, src/WellPeaceman.cpp:15-105.) */
+int  ocpx_set_wells(ocpx_ctx* ctx, int32_t nwell, const ocpx_well* wells, const int64_t* perf_cell,
+                    const double* perf_WI, const double* perf_dG, const double* perf_xi);
+/* ocpx_well_terms: the wells' part of IsoT_FIM::CalRes and AssembleMat
+ * (PeacemanWell::CalTrans + CalFlux, PeacemanWellIsoT::CalResFIM and
+ * AssembleMatInjFIM / AssembleMatProdFIM, src/WellPeaceman.cpp:365-462,
+ * 1226-1612) on the current state.  Call it once after ocpx_cal_res
+ * (jacobian = 0) or ocpx_assemble (jacobian = 1) with the same dt.  It adds
+ * the perforation rates to the bulk residual and right-hand side and
+ * recomputes the norms (norms may be NULL); with jacobian = 1 it adds the
+ * perforation terms to the bulk diagonal blocks and forms the well blocks
+ * (ocpx_get_well_blocks).  well_res [nwell] (may be NULL): each well row's
+ * residual, bhp - tarBHP or +-tarRate + sum_i q_i factor_i; the
+ * right-hand side of a well row is its negative. */
+int  ocpx_well_terms(ocpx_ctx* ctx, double dt, int32_t jacobian, ocpx_resnorms* norms, double* well_res);
+/* The blocks the reference's OCPMatrix gets from the wells beyond the bulk
+ * diagonal: Bnw [nperf][b] column 0 of block (perforated cell, well),
+ * Cwn [nperf][b] row 0 of block (well, perforated cell), Dw [nwell][b][b]
+ * the well's diagonal block.  The other entries of those blocks are zero.
+ * In OCPMatrix push order a perforated row's well columns follow its bulk
+ * columns, and a well row holds its diagonal and then its perforated cells
+ * in perforation order (well rows after the nb bulk rows). */
+int  ocpx_get_well_blocks(ocpx_ctx* ctx, double* Bnw, double* Cwn, double* Dw);
+/* bhp [nwell] and the component rates qw [nwell][nc] of the last
+ * ocpx_well_terms (either may be NULL); ocpx_set_well_bhp sets the
+ * bottom-hole pressures (PeacemanWellIsoT::GetSolutionFIM adds u of the
+ * well row to bhp, src/WellPeaceman.cpp:1198-1204). */
+int  ocpx_get_wells(ocpx_ctx* ctx, double* bhp, double* qw);
+int  ocpx_set_well_bhp(ocpx_ctx* ctx, const double* bhp);
+/* Peaceman well index of one perforation, PeacemanWell::CalWI
+ * (src/WellPeaceman.cpp:277-362): dir 0/1/2 = x/y/z, dz already times NTG,
+ * kh < 0 for the geometric kh; METRIC units, CONV_DARCY included.  Host
+ * function, no context.  Returns -1 for an unknown direction. */
+double ocpx_peaceman_wi(int32_t dir, double dx, double dy, double dz, double kx, double ky, double kz,
+                        double radius, double skin, double kh);
 
 /* ≙ LinearSystem::AssembleMatLinearSolver + Solve (src/LinearSystem.cpp:96-119)
  * as IsoT_FIM::SolveLinearSystem calls them (src/IsoThermalMethod.cpp:754-

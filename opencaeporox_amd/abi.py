@@ -51,6 +51,12 @@ class Grid(C.Structure):
                 ("conn_b", C.c_void_p), ("conn_e", C.c_void_p), ("trans", C.c_void_p)]
 
 
+class Well(C.Structure):
+    _fields_ = [("type", C.c_int32), ("mode", C.c_int32), ("perf0", C.c_int32), ("nperf", C.c_int32),
+                ("bhp", C.c_double), ("target", C.c_double),
+                ("zi", C.c_double * (MAX_NC + 1)), ("factor", C.c_double * (MAX_NC + 1))]
+
+
 class ResNorms(C.Structure):
     _fields_ = [("maxRelRes_V", C.c_double), ("maxId_V", C.c_int64),
                 ("maxRelRes_N", C.c_double), ("maxId_N", C.c_int64)]
@@ -128,6 +134,12 @@ def load(path=None):
         "ocpx_get_flash_queue_order": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_timing_enable": (C.c_int, [vp, C.c_int]),
         "ocpx_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(dbl), C.POINTER(i64)]),
+        "ocpx_set_wells": (C.c_int, [vp, i32, vp, vp, vp, vp, vp]),
+        "ocpx_well_terms": (C.c_int, [vp, dbl, i32, C.POINTER(ResNorms), vp]),
+        "ocpx_get_well_blocks": (C.c_int, [vp, vp, vp, vp]),
+        "ocpx_get_wells": (C.c_int, [vp, vp, vp]),
+        "ocpx_set_well_bhp": (C.c_int, [vp, vp]),
+        "ocpx_peaceman_wi": (dbl, [i32, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -499,6 +499,27 @@ hipError_t run_krpc_rock(const DevSat* s, const DevRock* r, const CellIO& c, hip
 #undef X_
 }
 
+// Norms of a residual already in resAbs (-R), as the norm loop of
+// IsoT_FIM::CalRes (src/IsoThermalMethod.cpp:1147-1168): used after terms
+// are added to the bulk rows (wells).
+template <int NCB>
+__global__ void __launch_bounds__(64) res_norms(CellIO c, const double* __restrict__ resAbs,
+                                                double* __restrict__ bmax, int64_t* __restrict__ barg)
+{
+    constexpr int B = NCB + 1, G = 64 / B;
+    const int lane = threadIdx.x;
+    const int g = lane / B, i = lane % B;
+    const int64_t n = (int64_t)blockIdx.x * G + g;
+    const bool valid = (g < G) && (n < c.nb);
+    double absV = -1, absN = -1;
+    if (valid) {
+        const double r = resAbs[(size_t)n * B + i];
+        absV = fabs(r / c.rockVp[n]);
+        if (i > 0) absN = fabs(r / c.Nt[n]);
+    }
+    block_norms(absV, absN, valid ? n : INT64_MAX, bmax, barg);
+}
+
 template <int NCB>
 static hipError_t launch_res_asm(const CellIO& c, const ConnIO& cn, const BsrIO& bsr, double dt, bool dj,
                                  double* res, double* bmax, int64_t* barg, hipStream_t st)
@@ -525,6 +546,15 @@ hipError_t run_res_asm(int nc, const CellIO& c, const ConnIO& cn, const BsrIO& b
     if (!dr) return hipErrorInvalidValue;      // the Jacobian pass always forms the residual too
     if (dj && !c.aux) return hipErrorInvalidValue;
 #define X(N) case N: return launch_res_asm<N>(c, cn, bsr, dt, dj, res, bmax, barg, st);
+    OCPX_NCB_SWITCH(X)
+#undef X
+}
+
+hipError_t run_res_norms(int nc, const CellIO& c, const double* res, double* bmax, int64_t* barg, hipStream_t st)
+{
+    if (c.nb == 0) return hipSuccess;
+    const unsigned nblk = (unsigned)res_asm_blocks(nc, c.nb);
+#define X(N) case N: hipLaunchKernelGGL(res_norms<N>, dim3(nblk), dim3(64), 0, st, c, res, bmax, barg); return hipGetLastError();
     OCPX_NCB_SWITCH(X)
 #undef X
 }

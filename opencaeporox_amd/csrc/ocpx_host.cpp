@@ -86,6 +86,12 @@ struct ocpx_ctx {
     int pe_lanes = 1;            // batched-flash mapping: lanes per cell (1, 4, 8, 16)
     int pe_lanes_auto = 0;       // the autotuner's choice, 0 if it has not run
     int team_blocks[3] = {0, 0, 0}; // persistent grid of the team kernels, T = 4 / 8 / 16
+    // Peaceman wells (ocpx_set_wells)
+    int32_t nwell = 0, nperf = 0;
+    ocpx_well* wells = nullptr;  // [nwell] device copy
+    int64_t* pcell = nullptr;
+    double *pWI = nullptr, *pdG = nullptr, *pxi = nullptr;
+    double *Bnw = nullptr, *Cwn = nullptr, *Dw = nullptr, *wres = nullptr, *qw = nullptr;
     // timing
     bool timing = false;
     TimingAcc tacc[9];
@@ -866,6 +872,159 @@ int ocpx_cal_res(ocpx_ctx* c, double dt, double* resAbs, ocpx_resnorms* norms)
         norms->maxRelRes_N = h[2]; norms->maxId_N = (int64_t)h[3];
     }
     return OCPX_OK;
+}
+
+namespace {
+
+void dfree(ocpx_ctx* c, void* p)
+{
+    if (!p) return;
+    for (auto it = c->allocs.begin(); it != c->allocs.end(); ++it)
+        if (*it == p) { c->allocs.erase(it); break; }
+    hipFree(p);
+}
+
+WellIO well_io(ocpx_ctx* c)
+{
+    WellIO w{};
+    w.nwell = c->nwell; w.wells = c->wells; w.cell = c->pcell; w.WI = c->pWI; w.dG = c->pdG; w.xi = c->pxi;
+    w.Bnw = c->Bnw; w.Cwn = c->Cwn; w.Dw = c->Dw; w.wres = c->wres; w.qw = c->qw;
+    return w;
+}
+
+}  // namespace
+
+int ocpx_set_wells(ocpx_ctx* c, int32_t nwell, const ocpx_well* wells, const int64_t* perf_cell,
+                   const double* perf_WI, const double* perf_dG, const double* perf_xi)
+{
+    if (!c) return OCPX_E_ARG;
+    if (nwell < 0 || (nwell > 0 && (!wells || !perf_cell || !perf_WI || !perf_dG)))
+        return fail(c, OCPX_E_ARG, "ocpx_set_wells: null well or perforation arrays");
+    int64_t np_ = 0;
+    std::vector<std::pair<int64_t, int32_t>> seen;
+    for (int32_t w = 0; w < nwell; w++) {
+        const ocpx_well& W = wells[w];
+        if ((W.type != OCPX_WELL_INJ && W.type != OCPX_WELL_PROD) || (W.mode != OCPX_WELL_BHP && W.mode != OCPX_WELL_RATE))
+            return fail(c, OCPX_E_ARG, "ocpx_set_wells: well " + std::to_string(w) + " has an unknown type or mode");
+        if (W.perf0 < 0 || W.nperf < 1)
+            return fail(c, OCPX_E_ARG, "ocpx_set_wells: well " + std::to_string(w) + " has no perforation");
+        if (W.type == OCPX_WELL_INJ && !perf_xi)
+            return fail(c, OCPX_E_ARG, "ocpx_set_wells: injector " + std::to_string(w) + " needs perf_xi");
+        np_ = std::max<int64_t>(np_, (int64_t)W.perf0 + W.nperf);
+        for (int32_t p = W.perf0; p < W.perf0 + W.nperf; p++) {
+            if (perf_cell[p] < 0 || perf_cell[p] >= c->nb)
+                return fail(c, OCPX_E_ARG, "ocpx_set_wells: perforation " + std::to_string(p) + " outside the grid");
+            seen.push_back({perf_cell[p], w});
+        }
+    }
+    std::sort(seen.begin(), seen.end());
+    for (size_t k = 1; k < seen.size(); k++)
+        if (seen[k] == seen[k - 1])
+            return fail(c, OCPX_E_ARG, "ocpx_set_wells: well " + std::to_string(seen[k].second) +
+                                           " perforates cell " + std::to_string(seen[k].first) + " twice");
+    void* old[10] = {c->wells, c->pcell, c->pWI, c->pdG, c->pxi, c->Bnw, c->Cwn, c->Dw, c->wres, c->qw};
+    for (void* p : old) dfree(c, p);
+    c->wells = nullptr; c->pcell = nullptr; c->pWI = c->pdG = c->pxi = nullptr;
+    c->Bnw = c->Cwn = c->Dw = c->wres = c->qw = nullptr;
+    c->nwell = 0; c->nperf = 0;
+    if (nwell == 0) return OCPX_OK;
+    const int B = c->nc + 1;
+    int rc;
+    if ((rc = dalloc(c, &c->wells, nwell)) || (rc = dalloc(c, &c->pcell, np_)) || (rc = dalloc(c, &c->pWI, np_)) ||
+        (rc = dalloc(c, &c->pdG, np_)) || (rc = dalloc(c, &c->pxi, np_)) || (rc = dalloc(c, &c->Bnw, np_ * B)) ||
+        (rc = dalloc(c, &c->Cwn, np_ * B)) || (rc = dalloc(c, &c->Dw, (size_t)nwell * B * B)) ||
+        (rc = dalloc(c, &c->wres, nwell)) || (rc = dalloc(c, &c->qw, (size_t)nwell * c->nc)))
+        return rc;
+    HIPCHK(c, hipMemcpy(c->wells, wells, sizeof(ocpx_well) * nwell, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->pcell, perf_cell, 8 * np_, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->pWI, perf_WI, 8 * np_, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->pdG, perf_dG, 8 * np_, hipMemcpyHostToDevice));
+    if (perf_xi) HIPCHK(c, hipMemcpy(c->pxi, perf_xi, 8 * np_, hipMemcpyHostToDevice));
+    c->nwell = nwell;
+    c->nperf = (int32_t)np_;
+    return OCPX_OK;
+}
+
+int ocpx_well_terms(ocpx_ctx* c, double dt, int32_t jacobian, ocpx_resnorms* norms, double* well_res)
+{
+    if (!c) return OCPX_E_ARG;
+    if (c->nwell == 0) return fail(c, OCPX_E_STATE, "ocpx_well_terms: no wells (ocpx_set_wells)");
+    if (jacobian && !c->assembled) return fail(c, OCPX_E_STATE, "ocpx_well_terms: the Jacobian needs ocpx_assemble first");
+    HIPCHK(c, run_well_terms(c->nc, cell_io(c), bsr_io(c), well_io(c), dt, jacobian != 0, c->resAbs, c->st));
+    HIPCHK(c, run_res_norms(c->nc, cell_io(c), c->resAbs, c->bmax, c->barg, c->st));
+    HIPCHK(c, run_norms_final(res_asm_blocks(c->nc, c->nb), c->bmax, c->barg, c->norms4, c->st));
+    if (norms || well_res) HIPCHK(c, hipStreamSynchronize(c->st));
+    if (norms) {
+        double h[4];
+        HIPCHK(c, hipMemcpy(h, c->norms4, sizeof(h), hipMemcpyDeviceToHost));
+        norms->maxRelRes_V = h[0]; norms->maxId_V = (int64_t)h[1];
+        norms->maxRelRes_N = h[2]; norms->maxId_N = (int64_t)h[3];
+    }
+    if (well_res) HIPCHK(c, hipMemcpy(well_res, c->wres, 8 * (size_t)c->nwell, hipMemcpyDeviceToHost));
+    return OCPX_OK;
+}
+
+int ocpx_get_well_blocks(ocpx_ctx* c, double* Bnw, double* Cwn, double* Dw)
+{
+    if (!c) return OCPX_E_ARG;
+    const size_t B = c->nc + 1;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (Bnw && c->nperf) HIPCHK(c, hipMemcpy(Bnw, c->Bnw, 8 * B * c->nperf, hipMemcpyDeviceToHost));
+    if (Cwn && c->nperf) HIPCHK(c, hipMemcpy(Cwn, c->Cwn, 8 * B * c->nperf, hipMemcpyDeviceToHost));
+    if (Dw && c->nwell) HIPCHK(c, hipMemcpy(Dw, c->Dw, 8 * B * B * c->nwell, hipMemcpyDeviceToHost));
+    return OCPX_OK;
+}
+
+int ocpx_get_wells(ocpx_ctx* c, double* bhp, double* qw)
+{
+    if (!c) return OCPX_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (bhp && c->nwell) {
+        std::vector<ocpx_well> h(c->nwell);
+        HIPCHK(c, hipMemcpy(h.data(), c->wells, sizeof(ocpx_well) * c->nwell, hipMemcpyDeviceToHost));
+        for (int32_t w = 0; w < c->nwell; w++) bhp[w] = h[w].bhp;
+    }
+    if (qw && c->nwell) HIPCHK(c, hipMemcpy(qw, c->qw, 8 * (size_t)c->nwell * c->nc, hipMemcpyDeviceToHost));
+    return OCPX_OK;
+}
+
+int ocpx_set_well_bhp(ocpx_ctx* c, const double* bhp)
+{
+    if (!c || !bhp) return OCPX_E_ARG;
+    if (c->nwell == 0) return OCPX_OK;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<ocpx_well> h(c->nwell);
+    HIPCHK(c, hipMemcpy(h.data(), c->wells, sizeof(ocpx_well) * c->nwell, hipMemcpyDeviceToHost));
+    for (int32_t w = 0; w < c->nwell; w++) h[w].bhp = bhp[w];
+    HIPCHK(c, hipMemcpy(c->wells, h.data(), sizeof(ocpx_well) * c->nwell, hipMemcpyHostToDevice));
+    return OCPX_OK;
+}
+
+double ocpx_peaceman_wi(int32_t dir, double dx, double dy, double dz, double kx, double ky, double kz,
+                        double radius, double skin, double kh)
+{
+    // PeacemanWell::CalWI, src/WellPeaceman.cpp:277-362 (PI and CONV_DARCY as the reference's METRIC constants)
+    const double PI_ = 3.141592653, DARCY_ = 0.008527;
+    double ro;
+    if (dir == 0) {
+        const double kykz = ky * kz, ky_kz = ky / kz;
+        ro = 0.28 * pow((dy * dy * pow(1 / ky_kz, 0.5) + dz * dz * pow(ky_kz, 0.5)), 0.5);
+        ro /= (pow(ky_kz, 0.25) + pow(1 / ky_kz, 0.25));
+        if (kh < 0) kh = (dx * pow(kykz, 0.5));
+    } else if (dir == 1) {
+        const double kzkx = kz * kx, kz_kx = kz / kx;
+        ro = 0.28 * pow((dz * dz * pow(1 / kz_kx, 0.5) + dx * dx * pow(kz_kx, 0.5)), 0.5);
+        ro /= (pow(kz_kx, 0.25) + pow(1 / kz_kx, 0.25));
+        if (kh < 0) kh = (dy * pow(kzkx, 0.5));
+    } else if (dir == 2) {
+        const double kxky = kx * ky, kx_ky = kx / ky;
+        ro = 0.28 * pow((dx * dx * pow(1 / kx_ky, 0.5) + dy * dy * pow(kx_ky, 0.5)), 0.5);
+        ro /= (pow(kx_ky, 0.25) + pow(1 / kx_ky, 0.25));
+        if (kh < 0) kh = (dz * pow(kxky, 0.5));
+    } else {
+        return -1.0;
+    }
+    return (2 * PI_) * kh / (log(ro / radius) + skin) * DARCY_;
 }
 
 int ocpx_assemble(ocpx_ctx* c, double dt)

@@ -6,6 +6,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "../../include/ocpx.h"
 
 namespace ocpx {
 
@@ -122,6 +123,27 @@ hipError_t run_res_asm(int nc, const CellIO& c, const ConnIO& cn, const BsrIO& b
                        hipStream_t st);
 hipError_t run_norms_final(int64_t nblocks, const double* blockmax, const int64_t* blockarg,
                            double* out4, hipStream_t st);
+// norms of the residual in res (-R) into blockmax/blockarg (res_asm_blocks(nc, nb) blocks)
+hipError_t run_res_norms(int nc, const CellIO& c, const double* res, double* blockmax, int64_t* blockarg,
+                         hipStream_t st);
+
+// Peaceman wells (wells.hip): one workgroup walks the wells and their
+// perforations in order (the reference's update sequence, no atomics)
+struct WellIO {
+    int32_t nwell;
+    const ocpx_well* wells;     // [nwell] device copy; bhp is the current iterate
+    const int64_t* cell;        // [nperf] perforated cell
+    const double* WI;           // [nperf] WI * multiplier (CONV_DARCY included)
+    const double* dG;           // [nperf] perforation head
+    const double* xi;           // [nperf] injected fluid's molar density (injectors)
+    double* Bnw;                // [nperf][b] column 0 of block (n, w)
+    double* Cwn;                // [nperf][b] row 0 of block (w, n)
+    double* Dw;                 // [nwell][b][b]
+    double* wres;               // [nwell] well-row residual (before the sign flip)
+    double* qw;                 // [nwell][nc] well component rates
+};
+hipError_t run_well_terms(int nc, const CellIO& c, const BsrIO& bsr, const WellIO& w, double dt, bool do_jac,
+                          double* resAbs, hipStream_t st);
 hipError_t run_get_solution(int nc, double* P, double* Ni, double* S, double* xij,
                             const double* dXsdXp, const double* u, double dSmax, int64_t nb,
                             hipStream_t st);

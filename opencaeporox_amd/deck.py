@@ -499,6 +499,49 @@ class Deck:
         return W.Grid(nx, ny, nz, v, depth, get("PORO")[:self.nb].astype(float).copy(),
                       cb[order].astype(np.int32), ce[order].astype(np.int32), tr[order])
 
+    def well_perforations(self, name):
+        """Perforations of a well from WELSPECS and COMPDAT on the Cartesian
+        grid (WellParam, src/ParamWell.cpp:50-127; PeacemanWell::InputPerfo,
+        src/WellPeaceman.cpp:15-53): [(cell, WI)] with WI in METRIC, CONV_DARCY
+        included, by PeacemanWell::CalWI unless COMPDAT gives it (a given WI
+        is taken times CONV1 as the reference does, then converted)."""
+        from . import wells as WL
+        spec = [w for w in self.welspecs if w[0] == name]
+        if not spec:
+            raise DeckError(f"well {name} is not in WELSPECS")
+        I, J = int(spec[-1][2]), int(spec[-1][3])
+        nx, ny, nz = self.dimens
+        fi = self.field_units
+        L = FT if fi else 1.0
+        out = []
+        for rec in self.compdat:
+            pat = rec[0]
+            star = pat.find("*")
+            if not (name == pat or (star >= 0 and name[:star] == pat[:star])):
+                continue
+            v = rec + ["DEFAULT"] * (10 - len(rec))
+            v = ["DEFAULT" if x == "/" else x for x in v]
+            i, j = (I, J) if v[1] == "DEFAULT" or v[2] == "DEFAULT" else (int(v[1]), int(v[2]))
+            for k in range(int(v[3]), int(v[4]) + 1):
+                n = (k - 1) * nx * ny + (j - 1) * nx + (i - 1)
+                if v[5] != "DEFAULT":
+                    WI = _float(v[5], "COMPDAT") * (5.61458 * FT ** 3 / PSI if fi else 1.0)
+                else:
+                    diam = _float(v[6], "COMPDAT") if v[6] != "DEFAULT" else 1.0
+                    kh = _float(v[7], "COMPDAT") * L if v[7] != "DEFAULT" else -1.0
+                    skin = _float(v[8], "COMPDAT") if v[8] != "DEFAULT" else 0.0
+                    d = v[9].lower() if v[9] != "DEFAULT" else "z"
+                    a = lambda kw: self.arrays[kw][n]
+                    ntg = self.arrays["NTG"][n] if "NTG" in self.arrays else 1.0
+                    ky = a("PERMY") if "PERMY" in self.arrays else a("PERMX")
+                    kz = a("PERMZ") if "PERMZ" in self.arrays else a("PERMX")
+                    WI = WL.peaceman_wi(d, a("DX") * L, a("DY") * L, a("DZ") * L * ntg, a("PERMX"), ky, kz,
+                                        diam / 2 * L, skin, kh)
+                out.append((n, WI))
+        if not out:
+            raise DeckError(f"well {name} has no COMPDAT perforation")
+        return out
+
     def composition(self, depth_m, region=0):
         """Hydrocarbon mole fractions per cell from ZMFVD (depth-interpolated,
         constant outside the table, normalised)."""

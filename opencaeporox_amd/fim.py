@@ -6,7 +6,8 @@ and exposes the per-Newton-step entry points under the reference's names:
     CalFlash        IsoT_FIM::CalFlash + PassFlashValue  (src/IsoThermalMethod.cpp:1016-1071)
     CalKrPc         IsoT_FIM::CalKrPc                     (:1073-1088)
     CalRock         IsothermalMethod::CalRock             (:19-30)
-    CalRes          IsoT_FIM::CalRes                      (:1090-1185, no wells)
+    CalRes          IsoT_FIM::CalRes                      (:1090-1185, bulk part)
+    CalResWells     its well part (src/WellPeaceman.cpp:1226-1612, after CalRes / AssembleMatBulks)
     AssembleMatBulks IsoT_FIM::AssembleMatBulks + CopyRhs (:1187-1269, :742-752)
     GetSolution     IsoT_FIM::GetSolution                 (:1282-1418, bulk part)
     UpdateLastTimeStep / ResetToLastTimeStep              (:1420-1515)
@@ -228,6 +229,43 @@ class IsoTFIM:
 
     def ResetToLastTimeStep(self):
         self._chk(self.lib.ocpx_reset_to_last_step(self.h), "reset to last step")
+
+    # ---- Peaceman wells
+    def SetWells(self, wells):
+        """ocpx_set_wells from a list of wells.PeacemanWell."""
+        from . import wells as WL
+        arr, cell, WI, dG, xi = WL.pack(wells, self.nc)
+        self._wells = list(wells)
+        self._chk(self.lib.ocpx_set_wells(self.h, len(wells), C.cast(arr, C.c_void_p), cell.ctypes.data,
+                                          WI.ctypes.data, dG.ctypes.data, xi.ctypes.data), "set wells")
+
+    def CalResWells(self, dt, jacobian=False):
+        """Well terms on top of CalRes (jacobian=False) or AssembleMatBulks
+        (jacobian=True): returns (norms, well-row residuals)."""
+        nrm = abi.ResNorms()
+        wres = np.zeros(len(self._wells))
+        self._chk(self.lib.ocpx_well_terms(self.h, dt, 1 if jacobian else 0, C.byref(nrm), wres.ctypes.data),
+                  "well terms")
+        return np.array([nrm.maxRelRes_V, nrm.maxId_V, nrm.maxRelRes_N, nrm.maxId_N]), wres
+
+    def well_blocks(self):
+        """(Bnw [nperf, b], Cwn [nperf, b], Dw [nwell, b, b])."""
+        npf = sum(len(w.cells) for w in self._wells)
+        Bnw, Cwn = np.zeros((npf, self.bs)), np.zeros((npf, self.bs))
+        Dw = np.zeros((len(self._wells), self.bs, self.bs))
+        self._chk(self.lib.ocpx_get_well_blocks(self.h, Bnw.ctypes.data, Cwn.ctypes.data, Dw.ctypes.data),
+                  "well blocks")
+        return Bnw, Cwn, Dw
+
+    def wells_state(self):
+        """(bhp [nwell], component rates [nwell, nc]) of the last well terms."""
+        bhp, qw = np.zeros(len(self._wells)), np.zeros((len(self._wells), self.nc))
+        self._chk(self.lib.ocpx_get_wells(self.h, bhp.ctypes.data, qw.ctypes.data), "wells")
+        return bhp, qw
+
+    def set_well_bhp(self, bhp):
+        b = np.ascontiguousarray(bhp, float)
+        self._chk(self.lib.ocpx_set_well_bhp(self.h, b.ctypes.data), "well bhp")
 
     def newton_pass(self, dt, mode=BATCHED, graph=False):
         self._chk(self.lib.ocpx_newton_pass(self.h, dt, mode, 1 if graph else 0), "newton pass")

@@ -1580,6 +1580,180 @@ void orc_assemble(const ocpx_fluid* fin, const orc_bulk* b, int64_t nconn, const
     free(fill); free(cnt);
 }
 
+/* ---- Peaceman wells ---------------------------------------------------
+ * Per-iteration well terms of the FIM step: CalTrans (src/WellPeaceman.cpp:
+ * 365-406), CalFlux (:409-462), CalResFIM (:1226-1265) and
+ * AssembleMatInjFIM / AssembleMatProdFIM (:1268-1360, :1488-1612), on top of
+ * the bulk residual of orc_cal_res (res: -R, updated to -(R + well terms),
+ * norms recomputed over the bulk rows as :1147-1168) and, when rowptr is
+ * given, the bulk BSR of orc_assemble (diagonal blocks updated in place).
+ * Outputs: wres [nw] well-row residual (before the sign flip), qw [nw*nc]
+ * the well's component rates; Bnw [nperf*len] column 0 of block (n, w),
+ * Cwn [nperf*len] row 0 of block (w, n), Dw [nw*len*len] the well's
+ * diagonal block (every other entry of those blocks is zero). */
+void orc_wells(const ocpx_fluid* fin, const orc_bulk* b, double dt, int32_t nw, const ocpx_well* w,
+               const int64_t* pcell, const double* pWI, const double* pdG, const double* pxi,
+               double* res, double* norms, double* wres, double* qw,
+               const int64_t* rowptr, double* val, double* Bnw, double* Cwn, double* Dw)
+{
+    const int nc = fin->nc + 1, np = 3, n1 = nc + 1, n2 = np * nc + np, bs = n1 * n1;
+    const int64_t nb = b->nb;
+    double *pB = malloc(sizeof(double) * bs), *pW = malloc(sizeof(double) * bs);
+    double *sB = malloc(sizeof(double) * n1 * n2), *bm = malloc(sizeof(double) * bs);
+    for (int32_t iw = 0; iw < nw; iw++) {
+        const ocpx_well* W = &w[iw];
+        const int inj = W->type == OCPX_WELL_INJ;
+        double qwell[MB];
+        for (int i = 0; i < nc; i++) qwell[i] = 0;
+        double* D = Dw ? &Dw[(size_t)iw * bs] : NULL;
+        if (D) memset(D, 0, sizeof(double) * bs);
+        for (int32_t p = W->perf0; p < W->perf0 + W->nperf; p++) {
+            const int64_t n = pcell[p];
+            const double WI = pWI[p], perfP = W->bhp + pdG[p];          /* CalPerfP */
+            /* CalTrans */
+            double tj[3], tINJ = 0;
+            for (int j = 0; j < np; j++) {
+                const int64_t id = n * np + j;
+                tj[j] = 0;
+                if (b->phaseExist[id]) { tj[j] = WI * b->kr[id] / b->mu[id]; tINJ += tj[j]; }
+            }
+            /* CalFlux */
+            double qi[MB];
+            for (int i = 0; i < nc; i++) qi[i] = 0;
+            if (inj) {
+                const double qt = tINJ * (b->P[n] - perfP);
+                for (int i = 0; i < nc; i++) qi[i] = qt * pxi[p] * W->zi[i];
+            } else {
+                for (int j = 0; j < np; j++) {
+                    const int64_t id = n * np + j;
+                    if (!b->phaseExist[id]) continue;
+                    const double qj = tj[j] * (b->Pj[id] - perfP);
+                    for (int i = 0; i < nc; i++) qi[i] += qj * b->xi[id] * b->xij[id * nc + i];
+                }
+            }
+            /* CalResFIM: well to bulk (res holds -R) */
+            for (int i = 0; i < nc; i++) {
+                qwell[i] += qi[i];
+                res[n * n1 + 1 + i] = -(-res[n * n1 + 1 + i] + qi[i] * dt);
+            }
+            if (!rowptr) continue;
+            /* AssembleMat{Inj,Prod}FIM */
+            memset(pB, 0, sizeof(double) * bs);
+            memset(pW, 0, sizeof(double) * bs);
+            memset(sB, 0, sizeof(double) * n1 * n2);
+            for (int j = 0; j < np; j++) {
+                const int64_t id = n * np + j;
+                if (!b->phaseExist[id]) continue;
+                const double mu = b->mu[id], muP = b->muP[id];
+                if (inj) {
+                    const double dP = b->P[n] - W->bhp - pdG[p];
+                    for (int i = 0; i < nc; i++) {
+                        const double tIJ = tj[j] * pxi[p] * W->zi[i];
+                        pB[(i + 1) * n1] += tIJ * (1 - dP * muP / mu);
+                        pW[(i + 1) * n1] += -tIJ;
+                        for (int k = 0; k < np; k++)
+                            sB[(i + 1) * n2 + k] += WI * pxi[p] * W->zi[i] * b->dKrdS[id * np + k] * dP / mu;
+                        for (int k = 0; k < nc; k++)
+                            sB[(i + 1) * n2 + np + j * nc + k] += -tIJ * dP / mu * b->mux[id * nc + k];
+                    }
+                } else {
+                    const double dP = b->Pj[id] - W->bhp - pdG[p];
+                    const double xi = b->xi[id], xiP = b->xiP[id];
+                    for (int i = 0; i < nc; i++) {
+                        const double xij = b->xij[id * nc + i];
+                        const double tIJ = tj[j] * xi * xij;
+                        pB[(i + 1) * n1] += tIJ * (1 - dP * muP / mu) + dP * tj[j] * xij * xiP;
+                        pW[(i + 1) * n1] += -tIJ;
+                        for (int k = 0; k < np; k++) {
+                            double t = WI * dP / mu * xi * xij * b->dKrdS[id * np + k];
+                            t += tIJ * b->dPcdS[id * np + k];
+                            sB[(i + 1) * n2 + k] += t;
+                        }
+                        for (int k = 0; k < nc; k++)
+                            sB[(i + 1) * n2 + np + j * nc + k] +=
+                                dP * tj[j] * xij * (b->xix[id * nc + k] - xi / mu * b->mux[id * nc + k]);
+                        sB[(i + 1) * n2 + np + j * nc + i] += tj[j] * xi * dP;
+                    }
+                }
+            }
+            /* bulk - bulk: AddDiag(n, dt*(dQdXpB + dQdXsB*dXsdXp)) */
+            memcpy(bm, pB, sizeof(double) * bs);
+            gemm_dt(n1, n1, n2, dt, sB, &b->dSec_dPri[n * n1 * n2], bm);
+            double* d = &val[rowptr[n] * bs];
+            for (int k = 0; k < bs; k++) d[k] += bm[k];
+            /* bulk - well: dt * dQdXpW (column 0) */
+            for (int r = 0; r < n1; r++) Bnw[(size_t)p * n1 + r] = pW[r * n1] * dt;
+            if (W->mode == OCPX_WELL_RATE) {
+                double b00 = 0;
+                for (int i = 0; i < nc; i++) b00 += pW[(i + 1) * n1] * W->factor[i];
+                D[0] += b00;
+                for (int i = 0; i < nc; i++) D[(i + 1) * n1 + i + 1] += 1;
+                memcpy(bm, pB, sizeof(double) * bs);
+                gemm_dt(n1, n1, n2, 1.0, sB, &b->dSec_dPri[n * n1 * n2], bm);
+                double* c = &Cwn[(size_t)p * n1];
+                for (int k = 0; k < n1; k++) c[k] = 0;
+                for (int i = 0; i < nc; i++)
+                    for (int k = 0; k < n1; k++) c[k] += W->factor[i] * bm[(i + 1) * n1 + k];
+            } else {
+                for (int r = 0; r < n1; r++) D[r * n1 + r] += 1;
+                for (int k = 0; k < n1; k++) Cwn[(size_t)p * n1 + k] = 0;
+            }
+        }
+        double r0;
+        if (W->mode == OCPX_WELL_RATE) {
+            r0 = inj ? W->target : -W->target;
+            for (int i = 0; i < nc; i++) r0 += qwell[i] * W->factor[i];
+        } else {
+            r0 = W->bhp - W->target;
+        }
+        if (wres) wres[iw] = r0;
+        if (qw) for (int i = 0; i < nc; i++) qw[(size_t)iw * nc + i] = qwell[i];
+    }
+    free(pB); free(pW); free(sB); free(bm);
+    if (norms) {
+        double mV = 0, mN = 0;
+        int64_t iV = 0, iN = 0;
+        for (int64_t n = 0; n < nb; n++) {
+            for (int i = 0; i < n1; i++) {
+                double t = fabs(res[n * n1 + i] / b->rockVp[n]);
+                if (mV < t) { mV = t; iV = n; }
+            }
+            for (int i = 1; i < n1; i++) {
+                double t = fabs(res[n * n1 + i] / b->Nt[n]);
+                if (mN < t) { mN = t; iN = n; }
+            }
+        }
+        norms[0] = mV; norms[1] = (double)iV; norms[2] = mN; norms[3] = (double)iN;
+    }
+}
+
+/* Peaceman well index of a perforation, CalWI (src/WellPeaceman.cpp:277-362)
+ * for the x, y, z directions (0, 1, 2): dz is already multiplied by NTG;
+ * kh < 0 selects the geometric kh; the result includes CONV_DARCY. */
+double orc_peaceman_wi(int dir, double dx, double dy, double dz, double kx, double ky, double kz,
+                       double radius, double skin, double kh)
+{
+    double ro = 0;
+    if (dir == 0) {
+        const double kykz = ky * kz, ky_kz = ky / kz;
+        ro = 0.28 * pow((dy * dy * pow(1 / ky_kz, 0.5) + dz * dz * pow(ky_kz, 0.5)), 0.5);
+        ro /= (pow(ky_kz, 0.25) + pow(1 / ky_kz, 0.25));
+        if (kh < 0) kh = (dx * pow(kykz, 0.5));
+    } else if (dir == 1) {
+        const double kzkx = kz * kx, kz_kx = kz / kx;
+        ro = 0.28 * pow((dz * dz * pow(1 / kz_kx, 0.5) + dx * dx * pow(kz_kx, 0.5)), 0.5);
+        ro /= (pow(kz_kx, 0.25) + pow(1 / kz_kx, 0.25));
+        if (kh < 0) kh = (dy * pow(kzkx, 0.5));
+    } else {
+        const double kxky = kx * ky, kx_ky = kx / ky;
+        ro = 0.28 * pow((dx * dx * pow(1 / kx_ky, 0.5) + dy * dy * pow(kx_ky, 0.5)), 0.5);
+        ro /= (pow(kx_ky, 0.25) + pow(1 / kx_ky, 0.25));
+        if (kh < 0) kh = (dz * pow(kxky, 0.5));
+    }
+    double WI = (2 * PI_REF) * kh / (log(ro / radius) + skin);
+    return WI * DARCY;
+}
+
 /* IsoT_FIM::GetSolution src/IsoThermalMethod.cpp:1330-1400 (bulk part) */
 void orc_get_solution(const ocpx_fluid* fin, orc_bulk* b, const double* u, double dSmax)
 {

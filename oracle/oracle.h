@@ -88,6 +88,14 @@ int64_t orc_cartesian_conns(int nx, int ny, int nz, const double* dx, const doub
                             const double* kz, const double* ntg, int32_t* cb,
                             int32_t* ce, double* trans);
 
+/* --- Peaceman wells (src/WellPeaceman.cpp), see ocpx_oracle.c ---------- */
+void orc_wells(const ocpx_fluid* fin, const orc_bulk* b, double dt, int32_t nw, const ocpx_well* w,
+               const int64_t* pcell, const double* pWI, const double* pdG, const double* pxi,
+               double* res, double* norms, double* wres, double* qw,
+               const int64_t* rowptr, double* val, double* Bnw, double* Cwn, double* Dw);
+double orc_peaceman_wi(int dir, double dx, double dy, double dz, double kx, double ky, double kz,
+                       double radius, double skin, double kh);
+
 #ifdef __cplusplus
 }
 #endif

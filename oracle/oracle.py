@@ -125,6 +125,9 @@ class Oracle:
         L.orc_cal_res.argtypes = [vp, vp, i64, vp, vp, vp, dbl, vp, vp]
         L.orc_assemble.argtypes = [vp, vp, i64, vp, vp, vp, dbl, vp, vp, vp]
         L.orc_get_solution.argtypes = [vp, vp, vp, dbl]
+        L.orc_wells.argtypes = [vp, vp, dbl, C.c_int32] + [vp] * 14
+        L.orc_peaceman_wi.restype = dbl
+        L.orc_peaceman_wi.argtypes = [C.c_int] + [dbl] * 9
         L.orc_cartesian_conns.restype = i64
         L.orc_cartesian_conns.argtypes = [C.c_int] * 3 + [vp] * 10
 
@@ -220,6 +223,30 @@ class Oracle:
         self.lib.orc_assemble(C.byref(fl), C.byref(b), grid.nconn, _p(cb), _p(ce), _p(tr), dt,
                               _p(rowptr), _p(col), _p(val))
         return rowptr, col, val.reshape(nnz, bs, bs)
+
+    def wells(self, fl, st, dt, wells, res, A=None):
+        """orc_wells: res (from cal_res, -R) updated in place to include the
+        wells; with A = (rowptr, col, val) from assemble, val's diagonal
+        blocks updated in place and the well blocks returned.  Returns
+        (norms, wres, qw, Bnw, Cwn, Dw)."""
+        from opencaeporox_amd import wells as WL
+        b = st.struct()
+        nc, bs = fl.nc + 1, fl.nc + 2
+        arr, cell, WI, dG, xi = WL.pack(wells, nc)
+        nw, npf = len(wells), len(cell)
+        norms, wres, qw = np.zeros(4), np.zeros(nw), np.zeros((nw, nc))
+        Bnw = Cwn = Dw = rowptr = val = None
+        if A is not None:
+            rowptr, _, val = A
+            assert val.flags.c_contiguous
+            Bnw, Cwn, Dw = np.zeros((npf, bs)), np.zeros((npf, bs)), np.zeros((nw, bs, bs))
+        self.lib.orc_wells(C.byref(fl), C.byref(b), dt, nw, C.cast(arr, C.c_void_p), _p(cell), _p(WI), _p(dG),
+                           _p(xi), _p(res), _p(norms), _p(wres), _p(qw), _p(rowptr), _p(val), _p(Bnw),
+                           _p(Cwn), _p(Dw))
+        return norms, wres, qw, Bnw, Cwn, Dw
+
+    def peaceman_wi(self, d, dx, dy, dz, kx, ky, kz, radius, skin, kh):
+        return self.lib.orc_peaceman_wi(d, dx, dy, dz, kx, ky, kz, radius, skin, kh)
 
     def get_solution(self, fl, st, u, dSmax):
         b = st.struct(); u = np.ascontiguousarray(u, D)
