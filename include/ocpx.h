@@ -347,7 +347,11 @@ double ocpx_peaceman_wi(int32_t dir, double dx, double dy, double dz, double kx,
  * 200 iterations, data/spe5/bsr.fasp).  *iters = iterations, negative when
  * not converged (the reference's convention); *relres = final ||r||/||b||.
  * u (host [nb*(nc+1)], may be NULL) receives the solution, which also stays
- * on the device for ocpx_get_solution(ctx, NULL, dSmax).  Synchronous. */
+ * on the device for ocpx_get_solution(ctx, NULL, dSmax).  When
+ * ocpx_well_terms(jacobian = 1) ran after the last residual / assembly pass
+ * the system has the well rows too (AddDim, src/WellPeaceman.cpp:1285): u is
+ * then [(nb + nwell)*(nc+1)], and ocpx_get_solution(ctx, NULL, dSmax) also
+ * adds each well row's entry 0 to its bhp (GetSolutionFIM).  Synchronous. */
 int  ocpx_linear_solve(ocpx_ctx* ctx, double rtol, int32_t maxit, double* u, int32_t* iters,
                        double* relres);
 

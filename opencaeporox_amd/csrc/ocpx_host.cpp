@@ -92,6 +92,11 @@ struct ocpx_ctx {
     int64_t* pcell = nullptr;
     double *pWI = nullptr, *pdG = nullptr, *pxi = nullptr;
     double *Bnw = nullptr, *Cwn = nullptr, *Dw = nullptr, *wres = nullptr, *qw = nullptr;
+    double *wrhs = nullptr, *dinv_w = nullptr, *rhsf = nullptr;
+    int64_t* wrow = nullptr;     // [nwell + 1] 0, 1, ..: block-row pointer of the well diagonal blocks
+    bool wells_jac = false;      // the assembled system holds the well rows (ocpx_well_terms, jacobian = 1)
+    int32_t solved_wells = 0;    // well rows in the last ocpx_linear_solve
+    int64_t sol_cap = 0;         // length of the solver vectors
     // timing
     bool timing = false;
     TimingAcc tacc[9];
@@ -358,6 +363,7 @@ int launch_krpc_rock(ocpx_ctx* c)
 
 int launch_res_asm(ocpx_ctx* c, double dt, bool dr, bool dj)
 {
+    c->wells_jac = false;              // a new residual / Jacobian holds no well terms yet
     {
         TimerScope t(c, dj ? 4 : 3);       // the residual / assembly kernel alone
         HIPCHK(c, run_res_asm(c->nc, cell_io(c), conn_io(c), bsr_io(c), dt, dr, dj, c->resAbs, c->bmax, c->barg,
@@ -888,7 +894,7 @@ WellIO well_io(ocpx_ctx* c)
 {
     WellIO w{};
     w.nwell = c->nwell; w.wells = c->wells; w.cell = c->pcell; w.WI = c->pWI; w.dG = c->pdG; w.xi = c->pxi;
-    w.Bnw = c->Bnw; w.Cwn = c->Cwn; w.Dw = c->Dw; w.wres = c->wres; w.qw = c->qw;
+    w.Bnw = c->Bnw; w.Cwn = c->Cwn; w.Dw = c->Dw; w.wres = c->wres; w.qw = c->qw; w.wrhs = c->wrhs;
     return w;
 }
 
@@ -922,19 +928,30 @@ int ocpx_set_wells(ocpx_ctx* c, int32_t nwell, const ocpx_well* wells, const int
         if (seen[k] == seen[k - 1])
             return fail(c, OCPX_E_ARG, "ocpx_set_wells: well " + std::to_string(seen[k].second) +
                                            " perforates cell " + std::to_string(seen[k].first) + " twice");
-    void* old[10] = {c->wells, c->pcell, c->pWI, c->pdG, c->pxi, c->Bnw, c->Cwn, c->Dw, c->wres, c->qw};
+    void* old[13] = {c->wells, c->pcell, c->pWI, c->pdG, c->pxi, c->Bnw, c->Cwn, c->Dw, c->wres, c->qw,
+                     c->wrhs, c->dinv_w, c->wrow};
     for (void* p : old) dfree(c, p);
     c->wells = nullptr; c->pcell = nullptr; c->pWI = c->pdG = c->pxi = nullptr;
-    c->Bnw = c->Cwn = c->Dw = c->wres = c->qw = nullptr;
+    c->Bnw = c->Cwn = c->Dw = c->wres = c->qw = c->wrhs = c->dinv_w = nullptr;
+    c->wrow = nullptr;
     c->nwell = 0; c->nperf = 0;
+    c->wells_jac = false;
+    c->solved_wells = 0;
     if (nwell == 0) return OCPX_OK;
     const int B = c->nc + 1;
     int rc;
     if ((rc = dalloc(c, &c->wells, nwell)) || (rc = dalloc(c, &c->pcell, np_)) || (rc = dalloc(c, &c->pWI, np_)) ||
         (rc = dalloc(c, &c->pdG, np_)) || (rc = dalloc(c, &c->pxi, np_)) || (rc = dalloc(c, &c->Bnw, np_ * B)) ||
         (rc = dalloc(c, &c->Cwn, np_ * B)) || (rc = dalloc(c, &c->Dw, (size_t)nwell * B * B)) ||
-        (rc = dalloc(c, &c->wres, nwell)) || (rc = dalloc(c, &c->qw, (size_t)nwell * c->nc)))
+        (rc = dalloc(c, &c->wres, nwell)) || (rc = dalloc(c, &c->qw, (size_t)nwell * c->nc)) ||
+        (rc = dalloc(c, &c->wrhs, (size_t)nwell * B)) || (rc = dalloc(c, &c->dinv_w, (size_t)nwell * B * B)) ||
+        (rc = dalloc(c, &c->wrow, (size_t)nwell + 1)))
         return rc;
+    {
+        std::vector<int64_t> wr(nwell + 1);
+        for (int32_t w = 0; w <= nwell; w++) wr[w] = w;
+        HIPCHK(c, hipMemcpy(c->wrow, wr.data(), 8 * wr.size(), hipMemcpyHostToDevice));
+    }
     HIPCHK(c, hipMemcpy(c->wells, wells, sizeof(ocpx_well) * nwell, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->pcell, perf_cell, 8 * np_, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->pWI, perf_WI, 8 * np_, hipMemcpyHostToDevice));
@@ -951,6 +968,7 @@ int ocpx_well_terms(ocpx_ctx* c, double dt, int32_t jacobian, ocpx_resnorms* nor
     if (c->nwell == 0) return fail(c, OCPX_E_STATE, "ocpx_well_terms: no wells (ocpx_set_wells)");
     if (jacobian && !c->assembled) return fail(c, OCPX_E_STATE, "ocpx_well_terms: the Jacobian needs ocpx_assemble first");
     HIPCHK(c, run_well_terms(c->nc, cell_io(c), bsr_io(c), well_io(c), dt, jacobian != 0, c->resAbs, c->st));
+    if (jacobian) c->wells_jac = true;
     HIPCHK(c, run_res_norms(c->nc, cell_io(c), c->resAbs, c->bmax, c->barg, c->st));
     HIPCHK(c, run_norms_final(res_asm_blocks(c->nc, c->nb), c->bmax, c->barg, c->norms4, c->st));
     if (norms || well_res) HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1076,6 +1094,8 @@ int ocpx_get_solution(ocpx_ctx* c, const double* u, double dSmax)
     }
     HIPCHK(c, run_get_solution(c->nc, D(c, OCPX_F_P), D(c, OCPX_F_NI), D(c, OCPX_F_S), D(c, OCPX_F_XIJ),
                                D(c, OCPX_F_DSEC_DPRI), ud, dSmax, c->nb, c->st));
+    if (!u && c->solved_wells)
+        HIPCHK(c, run_well_bhp_update(c->wells, c->solved_wells, (int)B, c->nb, c->sol, c->st));
     c->aux_valid = false;
     c->solved = false;
     return OCPX_OK;
@@ -1089,18 +1109,27 @@ int ocpx_linear_solve(ocpx_ctx* c, double rtol, int32_t maxit, double* u, int32_
     if (!c || !(rtol > 0) || maxit <= 0) return OCPX_E_ARG;
     if (!c->assembled) return fail(c, OCPX_E_STATE, "ocpx_linear_solve needs ocpx_assemble");
     const int B = c->nc + 1;
-    const int64_t nb = c->nb, len = nb * B;
+    const int32_t nwj = c->wells_jac ? c->nwell : 0;        // well rows after the nb bulk rows
+    const int64_t nb = c->nb, len = (nb + nwj) * B;
     int rc = OCPX_OK;
-    if (!c->sol) {
-        if ((rc = dalloc(c, &c->sol, (size_t)len)) || (rc = dalloc(c, &c->dinv, (size_t)nb * B * B)) ||
-            (rc = dalloc(c, &c->dpart, (size_t)dot_scratch_len())) || (rc = dalloc(c, &c->dout, 1)))
+    if (!c->dinv) {
+        if ((rc = dalloc(c, &c->dinv, (size_t)nb * B * B)) || (rc = dalloc(c, &c->dpart, (size_t)dot_scratch_len())) ||
+            (rc = dalloc(c, &c->dout, 1)))
             return rc;
+    }
+    if (c->sol_cap < len) {
+        dfree(c, c->sol); dfree(c, c->rhsf);
+        for (auto& v : c->kv) { dfree(c, v); v = nullptr; }
+        c->sol = c->rhsf = nullptr;
+        if ((rc = dalloc(c, &c->sol, (size_t)len)) || (rc = dalloc(c, &c->rhsf, (size_t)len))) return rc;
         for (auto& v : c->kv)
             if ((rc = dalloc(c, &v, (size_t)len))) return rc;
+        c->sol_cap = len;
     }
     double *r = c->kv[0], *rh = c->kv[1], *p = c->kv[2], *ph = c->kv[3], *v = c->kv[4], *s = c->kv[5],
            *sh = c->kv[6], *t = c->kv[7], *x = c->sol;
     const BsrIO io = bsr_io(c);
+    const WellIO wio = well_io(c);
     hipStream_t st = c->st;
     TimerScope whole(c, 7);
     auto dot = [&](const double* a, const double* b, double& out) -> int {
@@ -1112,20 +1141,29 @@ int ocpx_linear_solve(ocpx_ctx* c, double rtol, int32_t maxit, double* u, int32_
     auto spmv = [&](const double* in, double* outv) -> int {
         TimerScope ts(c, 8);
         HIPCHK(c, run_bsr_spmv(B, io.rowptr, io.colind, io.val, in, outv, nb, st));
+        if (nwj) HIPCHK(c, run_well_spmv(wio, B, nb, in, outv, st));
         return OCPX_OK;
     };
     auto precond = [&](const double* in, double* outv) -> int {
         HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv, in, outv, nb, st));
+        if (nwj) HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv_w, in + nb * B, outv + nb * B, nwj, st));
         return OCPX_OK;
     };
+    const double* rhs = io.rhs;
+    if (nwj) {                                               // [bulk rhs; well rhs]
+        HIPCHK(c, hipMemcpyAsync(c->rhsf, io.rhs, (size_t)nb * B * 8, hipMemcpyDeviceToDevice, st));
+        HIPCHK(c, hipMemcpyAsync(c->rhsf + nb * B, c->wrhs, (size_t)nwj * B * 8, hipMemcpyDeviceToDevice, st));
+        HIPCHK(c, run_bsr_diag_inv(B, c->wrow, c->Dw, nwj, c->dinv_w, st));
+        rhs = c->rhsf;
+    }
     HIPCHK(c, run_bsr_diag_inv(B, io.rowptr, io.val, nb, c->dinv, st));
     HIPCHK(c, run_axpbypcz(x, nullptr, 0, nullptr, 0, nullptr, len, st));
-    HIPCHK(c, run_axpbypcz(r, io.rhs, 0, nullptr, 0, nullptr, len, st));
-    HIPCHK(c, run_axpbypcz(rh, io.rhs, 0, nullptr, 0, nullptr, len, st));
+    HIPCHK(c, run_axpbypcz(r, rhs, 0, nullptr, 0, nullptr, len, st));
+    HIPCHK(c, run_axpbypcz(rh, rhs, 0, nullptr, 0, nullptr, len, st));
     HIPCHK(c, run_axpbypcz(p, nullptr, 0, nullptr, 0, nullptr, len, st));
     HIPCHK(c, run_axpbypcz(v, nullptr, 0, nullptr, 0, nullptr, len, st));
     double bb = 0;
-    if ((rc = dot(io.rhs, io.rhs, bb))) return rc;
+    if ((rc = dot(rhs, rhs, bb))) return rc;
     const double bnorm = std::sqrt(bb);
     int it = 0;
     bool conv = bnorm == 0;
@@ -1166,6 +1204,7 @@ int ocpx_linear_solve(ocpx_ctx* c, double rtol, int32_t maxit, double* u, int32_
         HIPCHK(c, hipStreamSynchronize(st));
     }
     c->solved = true;
+    c->solved_wells = nwj;
     return OCPX_OK;
 }
 
@@ -1263,6 +1302,7 @@ int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
     }
     HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
     c->flashed = c->assembled = true;
+    c->wells_jac = false;
     return OCPX_OK;
 }
 

@@ -140,10 +140,15 @@ struct WellIO {
     double* Cwn;                // [nperf][b] row 0 of block (w, n)
     double* Dw;                 // [nwell][b][b]
     double* wres;               // [nwell] well-row residual (before the sign flip)
+    double* wrhs;               // [nwell][b] right-hand side of the well rows (-wres in entry 0)
     double* qw;                 // [nwell][nc] well component rates
 };
 hipError_t run_well_terms(int nc, const CellIO& c, const BsrIO& bsr, const WellIO& w, double dt, bool do_jac,
                           double* resAbs, hipStream_t st);
+// the well rows and columns of y = A x after the bulk bsr_spmv (x, y of length (nb + nwell) * B)
+hipError_t run_well_spmv(const WellIO& w, int B, int64_t nb, const double* x, double* y, hipStream_t st);
+// bhp += u[(nb + w) * B] for every well (device copy of the wells)
+hipError_t run_well_bhp_update(ocpx_well* wells, int32_t nwell, int B, int64_t nb, const double* u, hipStream_t st);
 hipError_t run_get_solution(int nc, double* P, double* Ni, double* S, double* xij,
                             const double* dXsdXp, const double* u, double dSmax, int64_t nb,
                             hipStream_t st);

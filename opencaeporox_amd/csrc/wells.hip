@@ -12,6 +12,8 @@
 // the lanes split the entries: component rates on lanes 0..nc-1, the
 // dQ/dXs matrix (b x np*b) and then the b x b block dt*(dQ/dXp + dQ/dXs *
 // dXs/dXp) one entry per lane.
+#include <algorithm>
+
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 
@@ -182,9 +184,57 @@ __global__ void __launch_bounds__(64) well_terms(CellIO c, BsrIO bsr, WellIO w, 
             }
             w.wres[iw] = r0;
         }
+        if (lane < B) w.wrhs[(size_t)iw * B + lane] = lane == 0 ? -w.wres[iw] : 0.0;
         if (lane < nc) w.qw[(size_t)iw * nc + lane] = sqw[lane];
         __syncthreads();
     }
+}
+
+// y[bulk] += (n, w) blocks x[well]; y[well] = Dw x[well] + (w, n) blocks x[bulk].
+// The bulk part of y comes from bsr_spmv before this kernel runs.  One
+// workgroup, wells and perforations in order, lane r = row r of a block.
+__global__ void __launch_bounds__(64) well_spmv(WellIO w, int B, int64_t nb, const double* __restrict__ x,
+                                                double* __restrict__ y)
+{
+    const int lane = threadIdx.x;
+    for (int iw = 0; iw < w.nwell; iw++) {
+        const ocpx_well& W = w.wells[iw];
+        const double* xw = x + (size_t)(nb + iw) * B;
+        double acc = 0;
+        if (lane < B)
+            for (int c = 0; c < B; c++) acc += w.Dw[((size_t)iw * B + lane) * B + c] * xw[c];
+        for (int p = W.perf0; p < W.perf0 + W.nperf; p++) {
+            const int64_t n = w.cell[p];
+            if (lane < B) y[(size_t)n * B + lane] += w.Bnw[(size_t)p * B + lane] * xw[0];
+            if (lane == 0)
+                for (int c = 0; c < B; c++) acc += w.Cwn[(size_t)p * B + c] * x[(size_t)n * B + c];
+        }
+        if (lane < B) y[(size_t)(nb + iw) * B + lane] = acc;
+    }
+}
+
+// PeacemanWellIsoT::GetSolutionFIM (src/WellPeaceman.cpp:1198-1204): bhp += u of the well row
+__global__ void __launch_bounds__(64) well_bhp_update(ocpx_well* wells, int32_t nwell, int B, int64_t nb,
+                                                      const double* __restrict__ u)
+{
+    const int iw = threadIdx.x;
+    if (iw < nwell) wells[iw].bhp += u[(size_t)(nb + iw) * B];
+}
+
+hipError_t run_well_spmv(const WellIO& w, int B, int64_t nb, const double* x, double* y, hipStream_t st)
+{
+    if (w.nwell == 0) return hipSuccess;
+    hipLaunchKernelGGL(well_spmv, dim3(1), dim3(64), 0, st, w, B, nb, x, y);
+    return hipGetLastError();
+}
+
+hipError_t run_well_bhp_update(ocpx_well* wells, int32_t nwell, int B, int64_t nb, const double* u, hipStream_t st)
+{
+    for (int32_t w0 = 0; w0 < nwell; w0 += 64) {
+        hipLaunchKernelGGL(well_bhp_update, dim3(1), dim3(64), 0, st, wells + w0, std::min<int32_t>(64, nwell - w0),
+                           B, nb + w0, u);
+    }
+    return hipGetLastError();
 }
 
 #define OCPX_NC_SWITCH(X)                                                                           \

@@ -177,12 +177,14 @@ class IsoTFIM:
         norms = abi.ResNorms()
         self._chk(self.lib.ocpx_cal_res(self.h, dt, res.ctypes.data if fetch else None,
                                         C.byref(norms) if fetch else None), "residual")
+        self._wells_in_system = 0
         if not fetch:
             return None
         return res, np.array([norms.maxRelRes_V, norms.maxId_V, norms.maxRelRes_N, norms.maxId_N])
 
     def AssembleMatBulks(self, dt):
         self._chk(self.lib.ocpx_assemble(self.h, dt), "assemble")
+        self._wells_in_system = 0
 
     def bsr(self):
         nnz = C.c_int64()
@@ -208,7 +210,8 @@ class IsoTFIM:
     def LinearSolve(self, rtol=1e-3, maxit=200, fetch=True):
         """Device BiCGStab + block Jacobi on the assembled system (ocpx_linear_solve).
         Returns (u or None, iterations (negative if not converged), ||r||/||b||)."""
-        u = np.zeros(self.nb * self.bs) if fetch else None
+        rows = self.nb + getattr(self, "_wells_in_system", 0)     # well rows after ocpx_well_terms
+        u = np.zeros(rows * self.bs) if fetch else None
         it, rr = C.c_int32(), C.c_double()
         self._chk(self.lib.ocpx_linear_solve(self.h, rtol, maxit, None if u is None else u.ctypes.data,
                                              C.byref(it), C.byref(rr)), "linear solve")
@@ -246,6 +249,8 @@ class IsoTFIM:
         wres = np.zeros(len(self._wells))
         self._chk(self.lib.ocpx_well_terms(self.h, dt, 1 if jacobian else 0, C.byref(nrm), wres.ctypes.data),
                   "well terms")
+        if jacobian:
+            self._wells_in_system = len(self._wells)
         return np.array([nrm.maxRelRes_V, nrm.maxId_V, nrm.maxRelRes_N, nrm.maxId_N]), wres
 
     def well_blocks(self):
@@ -269,6 +274,7 @@ class IsoTFIM:
 
     def newton_pass(self, dt, mode=BATCHED, graph=False):
         self._chk(self.lib.ocpx_newton_pass(self.h, dt, mode, 1 if graph else 0), "newton pass")
+        self._wells_in_system = 0
 
     # ---- differentiation
     def jt_lambda(self, lam):

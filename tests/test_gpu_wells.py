@@ -112,3 +112,86 @@ def test_set_wells_rejects_bad_input():
     with pytest.raises(OcpxError, match="no wells"):
         dev.CalResWells(DT)
     dev.close()
+
+
+def _newton_case():
+    """The hydrostatic grid of the device Newton test with a rate producer
+    (90% of its initial rate) and a BHP injector of the reservoir's own
+    composition 0.5 bar above the highest pressure it perforates.  With a
+    direct solve the oracle's Newton sequence on this step is max |R| 4.5e3,
+    28, 8.7e-2, 1.2e-4, 1.7e-8, the well rows falling to 2.5e-10."""
+    from tests.test_gpu_solver import _smooth_grid
+    from oracle.oracle import Oracle
+    wl = _smooth_grid()
+    nc = wl.Nc + 1
+    g = wl.grid
+    col = lambda i, j: [i + j * g.nx + k * g.nx * g.ny for k in range(g.nz)]
+    P = wl.state["P"]
+    Ni = wl.state["NI"].reshape(-1, nc)[0]
+    zi = np.zeros(nc)
+    zi[:nc - 1] = Ni[:nc - 1] / Ni[:nc - 1].sum()
+    top = float(P[col(7, 7)].max()) + 0.5
+    wells = [WL.PeacemanWell("P", WL.PROD, WL.RATE, bhp=float(P[col(0, 0)].min()) - 2, target=0.0, cells=col(0, 0),
+                             WI=[2.0] * g.nz, dG=list(0.07 * np.arange(g.nz)), factor=[0.05] * nc),
+             WL.PeacemanWell("I", WL.INJ, WL.BHP, bhp=top, target=top, cells=col(7, 7), WI=[2.0] * g.nz,
+                             dG=list(0.05 * np.arange(g.nz)), xi=[8.0] * g.nz, zi=list(zi))]
+    o = Oracle()
+    st = state_from_workload(wl)
+    o.cal_flash(wl.fluid, st); o.cal_krpc(wl.sat, st); o.cal_rock(wl.rock, st)
+    res, _ = o.cal_res(wl.fluid, st, wl.grid, DT)
+    _, _, qw, *_ = o.wells(wl.fluid, st, DT, wells, res)
+    wells[0].target = 0.9 * float(qw[0] @ np.array(wells[0].factor))
+    return wl, wells
+
+
+def test_linear_solve_with_well_rows():
+    import scipy.sparse.linalg as spla
+    from opencaeporox_amd.fim import BATCHED
+    wl, wells = _newton_case()
+    dev = IsoTFIM.from_workload(wl)
+    dev.SetWells(wells)
+    dev.newton_pass(DT, BATCHED)
+    _, wres = dev.CalResWells(DT, jacobian=True)
+    rp, col, val, rhs = dev.bsr()
+    Bd, Cd, Dd = dev.well_blocks()
+    J, f = WL.full_system(rp, col, val, rhs, wells, Bd, Cd, Dd, wres, dev.bs)
+    u, it, rr = dev.LinearSolve(rtol=1e-12, maxit=2000)
+    assert u.size == (dev.nb + len(wells)) * dev.bs
+    true = np.linalg.norm(f - J @ u) / np.linalg.norm(f)
+    ref = spla.spsolve(J.tocsc(), f)
+    err = np.linalg.norm(u - ref) / np.linalg.norm(ref)
+    print(f"with wells: {it} iterations, true residual {true:.2e}, |u - u_direct| / |u_direct| {err:.2e}")
+    assert it > 0 and true < 1e-11 and err < 1e-8
+    dev.close()
+
+
+def test_newton_iterations_with_wells_on_device():
+    """assemble -> well terms -> device solve -> device GetSolution (bulk
+    update and bhp += u of the well rows), repeated; the same iterations with
+    u through the host and ocpx_set_well_bhp give the same bits."""
+    from opencaeporox_amd.fim import BATCHED
+    wl, wells = _newton_case()
+    a, b = IsoTFIM.from_workload(wl), IsoTFIM.from_workload(wl)
+    a.SetWells(wells)
+    b.SetWells(wells)
+    nb, bs = a.nb, a.bs
+    bulk, wrow = [], []
+    for k in range(5):
+        a.newton_pass(DT, BATCHED)
+        b.newton_pass(DT, BATCHED)
+        _, wa = a.CalResWells(DT, jacobian=True)
+        _, wb = b.CalResWells(DT, jacobian=True)
+        bulk.append(np.abs(a.bsr()[3]).max())
+        wrow.append(np.abs(wa).max())
+        u, it, rr = a.LinearSolve(rtol=1e-10, maxit=2000)
+        assert it > 0
+        a.GetSolution(None, 0.2)
+        b.GetSolution(u[:nb * bs], 0.2)
+        b.set_well_bhp(b.wells_state()[0] + u[nb * bs::bs])
+        for f in ("P", "NI", "S"):
+            assert np.array_equal(a.get(f), b.get(f)), f"iteration {k} field {f}"
+        assert np.array_equal(a.wells_state()[0], b.wells_state()[0]), f"iteration {k} bhp"
+    print("max |R| bulk:", " ".join(f"{v:.2e}" for v in bulk), "| well rows:", " ".join(f"{v:.2e}" for v in wrow))
+    assert bulk[-1] < 1e-6 * bulk[0] and wrow[-1] < 1e-6 * max(wrow[0], 1.0)
+    a.close()
+    b.close()
