@@ -337,6 +337,14 @@ int  ocpx_set_well_bhp(ocpx_ctx* ctx, const double* bhp);
  * function, no context.  Returns -1 for an unknown direction. */
 double ocpx_peaceman_wi(int32_t dir, double dx, double dy, double dz, double kx, double ky, double kz,
                         double radius, double skin, double kh);
+/* Molar density of a single-phase fluid, the injected fluid's perf_xi
+ * (MixtureUnit::XiPhase -> OCPMixtureMethodComp01::CalXi,
+ * src/OCPMixtureMethodComp.cpp:775-786; evaluated per time step, as
+ * PeacemanWell::CalFluxInit does): water = 1 / Bw(P) from PVTW, otherwise
+ * 1 / Vm(P, T + 273.15, z) of the Peng-Robinson fluid (z over the Nc
+ * hydrocarbon components).  P in bar, T in degC.  Host function, no
+ * context.  Returns -1 on a bad argument. */
+double ocpx_xi_phase(const ocpx_fluid* fluid, double P, double T, const double* z, int32_t water);
 
 /* ≙ LinearSystem::AssembleMatLinearSolver + Solve (src/LinearSystem.cpp:96-119)
  * as IsoT_FIM::SolveLinearSystem calls them (src/IsoThermalMethod.cpp:754-

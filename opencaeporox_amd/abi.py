@@ -140,6 +140,7 @@ def load(path=None):
         "ocpx_get_wells": (C.c_int, [vp, vp, vp]),
         "ocpx_set_well_bhp": (C.c_int, [vp, vp]),
         "ocpx_peaceman_wi": (dbl, [i32, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl, dbl]),
+        "ocpx_xi_phase": (dbl, [C.POINTER(Fluid), dbl, dbl, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1045,6 +1045,46 @@ double ocpx_peaceman_wi(int32_t dir, double dx, double dy, double dz, double kx,
     return (2 * PI_) * kh / (log(ro / radius) + skin) * DARCY_;
 }
 
+}  // extern "C"
+
+namespace {
+template <int NC>
+double hc_vm(const DevFluid& f, double P, double T, const double* z)
+{
+    EosCell<NC> e;
+    EosMix<NC> m;
+    eos_cell<NC>(f, P, T, e);
+    eos_mix<NC>(f, e, z, m);
+    double v = m.Z * R_GAS * T / P;
+    for (int i = 0; i < NC; i++) v -= z[i] * f.vsh[i];
+    return v;
+}
+}  // namespace
+
+extern "C" {
+
+double ocpx_xi_phase(const ocpx_fluid* f, double P, double T, const double* z, int32_t water)
+{
+    // OCPMixtureMethodComp01::CalXi (src/OCPMixtureMethodComp.cpp:775-786)
+    if (!f || f->nc < 2 || f->nc > 12 || !(P > 0)) return -1.0;
+    if (water) {                                   // OCP_PVTW::CalXiW
+        const double b = f->PVTW[1] * (1 - f->PVTW[2] * (P - f->PVTW[0]));
+        return 1 / b;
+    }
+    if (!z) return -1.0;
+    DevFluid h;
+    build_dev_fluid(f, h);
+    const double Tk = T + CONV4;
+    double v;
+    switch (f->nc) {
+#define X(N) case N: v = hc_vm<N>(h, P, Tk, z); break;
+    X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#undef X
+    default: return -1.0;
+    }
+    return 1 / v;
+}
+
 int ocpx_assemble(ocpx_ctx* c, double dt)
 {
     if (!c) return OCPX_E_ARG;

@@ -63,6 +63,16 @@ def peaceman_wi(direction, dx, dy, dz, kx, ky, kz, radius, skin=0.0, kh=-1.0):
     return abi.load().ocpx_peaceman_wi(d, dx, dy, dz, kx, ky, kz, radius, skin, kh)
 
 
+def xi_phase(fluid, P, T, z=None, water=False):
+    """Molar density of the injected fluid at a perforation (ocpx_xi_phase):
+    P bar, T degC, z over the Nc hydrocarbon components."""
+    zz = None if z is None else np.ascontiguousarray(z, float)
+    v = abi.load().ocpx_xi_phase(C.byref(fluid), P, T, None if zz is None else zz.ctypes.data, 1 if water else 0)
+    if v < 0:
+        raise ValueError("ocpx_xi_phase: bad argument")
+    return v
+
+
 def full_system(rowptr, col, val, rhs, wells, Bnw, Cwn, Dw, wres, b):
     """The reference's OCPMatrix and right-hand side with the well rows
     (scalar CSR of size (nb + nwell) * b).  Well w is block row nb + w; a

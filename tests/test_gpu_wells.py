@@ -93,7 +93,8 @@ def test_spe5_deck_wells(oracle):
     wells = [WL.PeacemanWell("PROD1", WL.PROD, WL.RATE, bhp=float(P[pc]) - 20.0, target=12000 * 0.159,
                              cells=[pc], WI=[pwi], factor=list(np.full(nc, 0.12))),
              WL.PeacemanWell("I", WL.INJ, WL.BHP, bhp=float(P[ic]) + 30.0, target=float(P[ic]) + 30.0,
-                             cells=[ic], WI=[iwi], xi=[6.5], zi=list(zi))]
+                             cells=[ic], WI=[iwi], zi=list(zi),
+                             xi=[WL.xi_phase(wl.fluid, float(P[ic]) + 30.0, d.temperature_c(), zi[:nc - 1])])]
     dev = IsoTFIM.from_workload(wl)
     _check(dev, oracle, wl, wells, "spe5")
     dev.close()

@@ -159,3 +159,19 @@ def test_peaceman_index_closed_form(oracle):
                     (1, (30.0, 60.0, 4.0, 10.0, 20.0, 5.0, 0.08, -0.5, 7.0))):
         assert lib.ocpx_peaceman_wi(d, *args) == oracle.peaceman_wi(d, *args)
     assert lib.ocpx_peaceman_wi(5, *args) == -1.0
+
+
+def test_xi_phase_is_the_single_phase_molar_density(oracle):
+    """ocpx_xi_phase (the injected fluid's perf_xi) against the oracle's molar
+    volume (with a volume shift) and its water PVTW."""
+    fl = W.make_fluid(6, 101, skip_psa=True)
+    for i in range(6):
+        fl.Vshift[i] = 0.02 * (i - 2)
+    rng = np.random.Generator(np.random.PCG64(12))
+    for P, T in ((50.0, 60.0), (150.0, 71.0), (400.0, 120.0)):
+        z = rng.dirichlet(np.ones(6))
+        vm = oracle.vmder(fl, P, T + W.CONV4, z)[0]
+        assert WL.xi_phase(fl, P, T, z) == pytest.approx(1 / vm, rel=1e-13)
+        assert WL.xi_phase(fl, P, T, water=True) == pytest.approx(oracle.pvtw(fl, P)[1], rel=1e-14)
+    with pytest.raises(ValueError):
+        WL.xi_phase(fl, -1.0, 60.0, z)
