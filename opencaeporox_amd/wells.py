@@ -73,6 +73,64 @@ def xi_phase(fluid, P, T, z=None, water=False):
     return v
 
 
+PSURF, TSURF = 1.01325, 20.0          # METRIC standard conditions (include/OCPUnits.hpp:97-99)
+
+
+def rate_factors(fluid, qi, weights, Psurf=PSURF, Tsurf=TSURF):
+    """Rate-equation weights of producers, PeacemanWell::CalFactor
+    (src/WellPeaceman.cpp:654-718) for compositional fluids: the produced
+    moles qi [nwell, nc] are flashed at (Psurf, Tsurf) on the device
+    (CalVStd), the surface volumes of oil, gas and water are weighted by
+    weights [nwell, 3] (prodPhaseWeight: ORAT (1, 0, 0), GRAT (0, 1, 0),
+    WRAT (0, 0, 1), LRAT (1, 0, 1)), and every component gets
+    factor = qv / qt.  METRIC: the surface-unit conversions are 1 and water's
+    surface volume is its moles (stdVw = 1).  Requires qi >= 0 with
+    sum > 1e-8 (the reference then falls back to perforation mobilities)."""
+    from .fim import PLAIN, IsoTFIM
+    qi = np.atleast_2d(np.asarray(qi, float))
+    weights = np.atleast_2d(np.asarray(weights, float))
+    nw, nc = qi.shape
+    if nc != fluid.nc + 1 or weights.shape != (nw, 3):
+        raise ValueError("qi must be [nwell, Nc + 1] and weights [nwell, 3]")
+    qt = qi.sum(axis=1)
+    if np.any(qi < 0) or np.any(qt <= 1e-8):
+        raise ValueError("rate_factors needs non-negative produced moles")
+    wl = surface_workload(fluid, qi, Psurf, Tsurf)
+    dev = IsoTFIM.from_workload(wl)
+    try:
+        dev.CalFlash(PLAIN)
+        S = dev.get("S")
+        vf = dev.get("VF")
+    finally:
+        dev.close()
+    return factors_from_flash(S, vf, qi, weights)
+
+
+def surface_workload(fluid, qi, Psurf=PSURF, Tsurf=TSURF):
+    """One cell per well holding the produced moles at (Psurf, Tsurf)."""
+    from . import workload as W
+    nw, nc = qi.shape
+    fl = abi.Fluid.from_buffer_copy(fluid)
+    fl.skipPSA = 0
+    grid = W.Grid(nw, 1, 1, np.ones(nw), np.zeros(nw), np.full(nw, 0.2), np.zeros(0, np.int32),
+                  np.zeros(0, np.int32), np.zeros(0))
+    st = {"P": np.full(nw, float(Psurf)), "T": np.full(nw, float(Tsurf)), "NI": qi.ravel().copy(),
+          "LNI": qi.ravel().copy(), "PHASEEXIST": np.tile(np.array([0, 0, 1], np.uint32), nw),
+          "S": np.tile(np.array([0.0, 0.0, 1.0]), nw), "XIJ": np.zeros(nw * 3 * nc)}
+    st["XIJ"].reshape(nw, 3, nc)[:, 2, nc - 1] = 1.0
+    return W.Workload("surface", fl, W.make_sat(), W.make_rock(), grid, st)
+
+
+def factors_from_flash(S, vf, qi, weights):
+    """qv / qt from a surface flash: oil and gas volumes S_j * Vf, water's
+    surface volume its moles."""
+    nw, nc = qi.shape
+    S = np.asarray(S).reshape(nw, 3)
+    vj = S[:, :2] * np.asarray(vf)[:, None]
+    qv = (vj * weights[:, :2]).sum(axis=1) + qi[:, nc - 1] * weights[:, 2]
+    return qv / qi.sum(axis=1)
+
+
 def full_system(rowptr, col, val, rhs, wells, Bnw, Cwn, Dw, wres, b):
     """The reference's OCPMatrix and right-hand side with the well rows
     (scalar CSR of size (nb + nwell) * b).  Well w is block row nb + w; a

@@ -196,3 +196,23 @@ def test_newton_iterations_with_wells_on_device():
     assert bulk[-1] < 1e-6 * bulk[0] and wrow[-1] < 1e-6 * max(wrow[0], 1.0)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("name", ["spe5", "spe3"])
+def test_rate_factors_match_oracle(oracle, name):
+    """CalFactor: the produced stream flashed at surface conditions on the
+    device against the oracle's flash of the same moles."""
+    d = deck.read_deck(os.path.join(DECKS, name, f"{name}.data"))
+    fl = d.to_fluid()
+    z = d.composition(np.array([2300.0]))[0]
+    qi = np.array([list(z * 100.0) + [5.0], list(z * 40.0) + [0.0], list(z * 7.0) + [20.0]])
+    w = np.array([[1, 0, 0], [0, 1, 0], [1, 0, 1]], float)       # ORAT, GRAT, LRAT
+    fd = WL.rate_factors(fl, qi, w)
+    wl = WL.surface_workload(fl, qi)
+    st = state_from_workload(wl)
+    oracle.cal_flash(wl.fluid, st)
+    fo = WL.factors_from_flash(st.a["S"], st.a["VF"], qi, w)
+    print(name, "factors", fd)
+    assert np.all(fd > 0)
+    ok, worst = close(fd, fo)
+    assert ok, f"{name}: worst {worst}"

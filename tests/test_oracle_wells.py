@@ -175,3 +175,11 @@ def test_xi_phase_is_the_single_phase_molar_density(oracle):
         assert WL.xi_phase(fl, P, T, water=True) == pytest.approx(oracle.pvtw(fl, P)[1], rel=1e-14)
     with pytest.raises(ValueError):
         WL.xi_phase(fl, -1.0, 60.0, z)
+
+
+def test_rate_factors_reject_bad_input():
+    fl = W.make_fluid(3, 101)
+    with pytest.raises(ValueError, match="non-negative"):
+        WL.rate_factors(fl, np.array([[1.0, -1.0, 0.5, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    with pytest.raises(ValueError, match="nwell"):
+        WL.rate_factors(fl, np.array([[1.0, 1.0, 0.5]]), np.array([[1.0, 0.0, 0.0]]))
