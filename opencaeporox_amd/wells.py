@@ -131,6 +131,140 @@ def factors_from_flash(S, vf, qi, weights):
     return qv / qi.sum(axis=1)
 
 
+class SegmentFlash:
+    """Flash of one mixture at (P, T) on the device (a one-cell context,
+    reused): returns (phase exists [3], phase volumes [3], densities [3]),
+    as PVT->Flash + GetVj / GetRho in the perforation-head loops."""
+
+    def __init__(self, fluid):
+        from .fim import IsoTFIM
+        nc = fluid.nc + 1
+        self.wl = surface_workload(fluid, np.ones((1, nc)))
+        self.dev = IsoTFIM.from_workload(self.wl)
+        self.base = {k: np.array(v) for k, v in self.wl.state.items()}
+
+    def __call__(self, P, T, Ni):
+        from .fim import PLAIN
+        d = self.dev
+        for k in ("PHASEEXIST", "S", "XIJ"):
+            d.set(k, self.base[k])
+        d.set("P", np.array([float(P)]))
+        d.set("T", np.array([float(T)]))
+        d.set("NI", np.asarray(Ni, float))
+        d.CalFlash(PLAIN)
+        S, vf = d.get("S"), d.get("VF")
+        return d.get("PHASEEXIST").astype(bool), S * vf[0], d.get("RHO")
+
+    def close(self):
+        self.dev.close()
+
+
+def _segments(d0, d1, maxlen):
+    n = int(np.ceil(abs((d1 - d0) / maxlen)))
+    return n, ((d1 - d0) / n if n else 0.0)
+
+
+def producer_heads(cell, well, well_depth, perf_depth, flash, maxlen=10.0):
+    """Perforation heads of a producer, PeacemanWell::CalProddG01
+    (src/WellPeaceman.cpp:802-923).  cell: dict of the perforated cells'
+    state (P, T [degC], PHASEEXIST [n, 3], KR, MU, XI [n, 3], XIJ [n, 3, nc],
+    NI [n, nc]) in perforation order; perforation pressures are bhp + the
+    current dG; the produced stream accumulates from the far end of the well
+    and is flashed once per segment of at most maxlen (deck length units,
+    10) with flash(P, T, Ni).  Returns the new dG; a well head between the
+    perforations leaves dG as it was (the reference does the same)."""
+    npf = len(well.cells)
+    dG_old = np.asarray(well.dG if well.dG is not None else np.zeros(npf), float)
+    perfP = well.bhp + dG_old
+    nc = cell["NI"].shape[1]
+    ex = cell["PHASEEXIST"]
+    tj = np.where(ex, np.asarray(well.WI)[:, None] * cell["KR"] / np.where(ex, cell["MU"], 1.0), 0.0)
+    g = 0.0000980665
+    dGp = np.zeros(npf)
+    tmpNi = np.zeros(nc)
+
+    def stream(p):
+        for j in range(3):
+            if cell["PHASEEXIST"][p, j]:
+                for k in range(nc):
+                    tmpNi[k] += (cell["P"][p] - perfP[p]) * tj[p, j] * cell["XI"][p, j] * cell["XIJ"][p, j, k]
+        if np.abs(tmpNi).sum() < 1e-8:                # no inflow yet: the cell's own moles
+            tmpNi[:] = cell["NI"][p]
+        return tmpNi
+
+    def run(p, n, h, sign):
+        Ni = stream(p)
+        Pt = perfP[p]
+        for _ in range(n):
+            ex, vj, rho = flash(Pt, cell["T"][p], Ni)
+            qt = vj[ex].sum()
+            Pt += sign * (vj[ex] * rho[ex]).sum() / qt * h * g
+        return Pt
+
+    if well_depth <= perf_depth[0]:                  # head above the perforations
+        for p in range(npf - 1, -1, -1):
+            n, h = _segments(well_depth if p == 0 else perf_depth[p - 1], perf_depth[p], maxlen)
+            if n == 0:                                 # the reference skips the perforation's inflow too
+                continue
+            dGp[p] = perfP[p] - run(p, n, h, -1.0)
+        return np.cumsum(dGp)
+    if well_depth >= perf_depth[-1]:                 # head below the perforations
+        for p in range(npf):
+            n, h = _segments(perf_depth[p], well_depth if p == npf - 1 else perf_depth[p + 1], maxlen)
+            if n == 0:
+                continue
+            dGp[p] = run(p, n, h, +1.0) - perfP[p]
+        return np.cumsum(dGp[::-1])[::-1]
+    return dG_old
+
+
+def injector_heads(fluid, well, well_depth, perf_depth, T, water=False, maxlen=10.0):
+    """Perforation heads of an injector, PeacemanWell::CalInjdG
+    (src/WellPeaceman.cpp:733-799): the injected fluid's density
+    (RhoPhase: MW * xi from ocpx_xi_phase, water rhoW_std / Bw) integrated
+    over segments of at most maxlen from each perforation pressure."""
+    npf = len(well.cells)
+    dG_old = np.asarray(well.dG if well.dG is not None else np.zeros(npf), float)
+    perfP = well.bhp + dG_old
+    Nc = fluid.nc
+    z = np.asarray(well.zi, float)[:Nc]
+    mw = sum(z[i] * fluid.MW[i] for i in range(Nc))
+    g = 0.0000980665
+
+    def rho(P):
+        return fluid.rhoW_std * xi_phase(fluid, P, T, water=True) if water else mw * xi_phase(fluid, P, T, z)
+
+    dGp = np.zeros(npf)
+    if well_depth <= perf_depth[0]:
+        for p in range(npf - 1, -1, -1):
+            n, h = _segments(well_depth if p == 0 else perf_depth[p - 1], perf_depth[p], maxlen)
+            Pt = perfP[p]
+            for _ in range(n):
+                Pt -= rho(Pt) * g * h
+            dGp[p] = perfP[p] - Pt
+        return np.cumsum(dGp)
+    if well_depth >= perf_depth[-1]:
+        for p in range(npf):
+            n, h = _segments(perf_depth[p], well_depth if p == npf - 1 else perf_depth[p + 1], maxlen)
+            Pt = perfP[p]
+            for _ in range(n):
+                Pt += rho(Pt) * g * h
+            dGp[p] = Pt - perfP[p]
+        return np.cumsum(dGp[::-1])[::-1]
+    return dG_old
+
+
+def cell_state(dev, cells):
+    """The perforated cells' fields producer_heads reads, from the device."""
+    nb, nc = dev.nb, dev.nc
+    c = np.asarray(cells)
+    return {"P": dev.get("P")[c], "T": dev.get("T")[c],
+            "PHASEEXIST": dev.get("PHASEEXIST").reshape(nb, 3)[c].astype(bool),
+            "KR": dev.get("KR").reshape(nb, 3)[c], "MU": dev.get("MU").reshape(nb, 3)[c],
+            "XI": dev.get("XI").reshape(nb, 3)[c], "XIJ": dev.get("XIJ").reshape(nb, 3, nc)[c],
+            "NI": dev.get("NI").reshape(nb, nc)[c]}
+
+
 def full_system(rowptr, col, val, rhs, wells, Bnw, Cwn, Dw, wres, b):
     """The reference's OCPMatrix and right-hand side with the well rows
     (scalar CSR of size (nb + nwell) * b).  Well w is block row nb + w; a

@@ -216,3 +216,41 @@ def test_rate_factors_match_oracle(oracle, name):
     assert np.all(fd > 0)
     ok, worst = close(fd, fo)
     assert ok, f"{name}: worst {worst}"
+
+
+def test_producer_heads_match_oracle(oracle):
+    """CalProddG01: the produced stream flashed segment by segment up the
+    well (device flash) against the same loop over the oracle's states and
+    flash; the heads grow with depth."""
+    from opencaeporox_amd.fim import BATCHED
+    wl, wells = _newton_case()
+    prod = wells[0]
+    nb, nc = wl.grid.nb, wl.Nc + 1
+    dev = IsoTFIM.from_workload(wl)
+    dev.CalFlash(BATCHED)
+    dev.CalKrPc()
+    st = state_from_workload(wl)
+    oracle.cal_flash(wl.fluid, st)
+    oracle.cal_krpc(wl.sat, st)
+    depth = wl.grid.depth[prod.cells]
+    head = float(depth[0]) - 25.0
+    seg = WL.SegmentFlash(wl.fluid)
+    dG_d = WL.producer_heads(WL.cell_state(dev, prod.cells), prod, head, depth, seg)
+    seg.close()
+    c = np.asarray(prod.cells)
+    cs = {"P": st.a["P"][c], "T": st.a["T"][c], "PHASEEXIST": st.a["PHASEEXIST"].reshape(nb, 3)[c].astype(bool),
+          "KR": st.a["KR"].reshape(nb, 3)[c], "MU": st.a["MU"].reshape(nb, 3)[c], "XI": st.a["XI"].reshape(nb, 3)[c],
+          "XIJ": st.a["XIJ"].reshape(nb, 3, nc)[c], "NI": st.a["NI"].reshape(nb, nc)[c]}
+
+    def oflash(P, T, Ni):
+        w2 = WL.surface_workload(wl.fluid, np.asarray(Ni, float)[None, :], P, T)
+        s2 = state_from_workload(w2)
+        oracle.cal_flash(w2.fluid, s2)
+        return s2.a["PHASEEXIST"].astype(bool), s2.a["S"] * s2.a["VF"][0], s2.a["RHO"]
+
+    dG_o = WL.producer_heads(cs, prod, head, depth, oflash)
+    print("producer heads (bar):", dG_d)
+    assert np.all(np.diff(dG_d) > 0) and dG_d[0] > 0
+    ok, worst = close(dG_d, dG_o)
+    assert ok, f"worst {worst}"
+    dev.close()

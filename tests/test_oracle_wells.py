@@ -183,3 +183,28 @@ def test_rate_factors_reject_bad_input():
         WL.rate_factors(fl, np.array([[1.0, -1.0, 0.5, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
     with pytest.raises(ValueError, match="nwell"):
         WL.rate_factors(fl, np.array([[1.0, 1.0, 0.5]]), np.array([[1.0, 0.0, 0.0]]))
+
+
+def test_injector_heads_integrate_the_fluid_column():
+    """CalInjdG: incompressible water gives dG = rho g (depth - head depth)
+    exactly; a gas injector's heads approach the solution of
+    dP/dz = rho(P) g as the segments shrink."""
+    from scipy.integrate import solve_ivp
+    fl = W.make_fluid(4, 101)
+    fl.PVTW[2] = 0.0                                  # incompressible water
+    depths = np.array([2010.0, 2030.0, 2055.0])
+    w = WL.PeacemanWell("I", WL.INJ, WL.BHP, bhp=200.0, target=200.0, cells=[0, 1, 2], WI=[1.0] * 3,
+                        dG=[0.0] * 3, zi=[0.0] * 5)
+    dG = WL.injector_heads(fl, w, 2000.0, depths, T=70.0, water=True)
+    rho = fl.rhoW_std / fl.PVTW[1]
+    assert np.allclose(dG, rho * 0.0000980665 * (depths - 2000.0), rtol=1e-12)
+    zi = [0.9, 0.06, 0.03, 0.01, 0.0]
+    w = WL.PeacemanWell("I", WL.INJ, WL.BHP, bhp=200.0, target=200.0, cells=[0, 1, 2], WI=[1.0] * 3,
+                        dG=[0.0] * 3, zi=zi)
+    mw = sum(zi[i] * fl.MW[i] for i in range(4))
+    ode = lambda z, P: [mw * WL.xi_phase(fl, P[0], 70.0, zi[:4]) * 0.0000980665]     # dP/dz = rho g
+    fine = WL.injector_heads(fl, w, 2000.0, depths, T=70.0, maxlen=0.05)
+    for p, d in enumerate(depths):
+        # each perforation integrates upwards from its own pressure (bhp + old dG = 200 bar)
+        top = solve_ivp(ode, (d, 2000.0 if p == 0 else depths[p - 1]), [200.0], rtol=1e-11, atol=1e-9).y[0, -1]
+        assert 200.0 - top == pytest.approx(np.diff(np.r_[0.0, fine])[p], rel=2e-5)
