@@ -93,6 +93,10 @@ struct ocpx_ctx {
     double *pWI = nullptr, *pdG = nullptr, *pxi = nullptr;
     double *Bnw = nullptr, *Cwn = nullptr, *Dw = nullptr, *wres = nullptr, *qw = nullptr;
     double *wrhs = nullptr, *dinv_w = nullptr, *rhsf = nullptr;
+    double *qperf = nullptr, *bdiag = nullptr, *dD = nullptr;
+    int32_t *pwell = nullptr, *uptr = nullptr, *uperf = nullptr;
+    int64_t* ucell = nullptr;
+    int32_t nucell = 0;
     int64_t* wrow = nullptr;     // [nwell + 1] 0, 1, ..: block-row pointer of the well diagonal blocks
     bool wells_jac = false;      // the assembled system holds the well rows (ocpx_well_terms, jacobian = 1)
     int32_t solved_wells = 0;    // well rows in the last ocpx_linear_solve
@@ -895,6 +899,8 @@ WellIO well_io(ocpx_ctx* c)
     WellIO w{};
     w.nwell = c->nwell; w.wells = c->wells; w.cell = c->pcell; w.WI = c->pWI; w.dG = c->pdG; w.xi = c->pxi;
     w.Bnw = c->Bnw; w.Cwn = c->Cwn; w.Dw = c->Dw; w.wres = c->wres; w.qw = c->qw; w.wrhs = c->wrhs;
+    w.nperf = c->nperf; w.nucell = c->nucell; w.pwell = c->pwell; w.ucell = c->ucell; w.uptr = c->uptr;
+    w.uperf = c->uperf; w.qperf = c->qperf; w.bdiag = c->bdiag; w.dD = c->dD;
     return w;
 }
 
@@ -923,16 +929,32 @@ int ocpx_set_wells(ocpx_ctx* c, int32_t nwell, const ocpx_well* wells, const int
             seen.push_back({perf_cell[p], w});
         }
     }
+    {
+        std::vector<int32_t> owner(np_, -1);
+        for (int32_t w = 0; w < nwell; w++)
+            for (int32_t p = wells[w].perf0; p < wells[w].perf0 + wells[w].nperf; p++) {
+                if (owner[p] >= 0)
+                    return fail(c, OCPX_E_ARG, "ocpx_set_wells: perforation " + std::to_string(p) + " in two wells");
+                owner[p] = w;
+            }
+        for (int64_t p = 0; p < np_; p++)
+            if (owner[p] < 0)
+                return fail(c, OCPX_E_ARG, "ocpx_set_wells: perforation " + std::to_string(p) + " in no well");
+    }
     std::sort(seen.begin(), seen.end());
     for (size_t k = 1; k < seen.size(); k++)
         if (seen[k] == seen[k - 1])
             return fail(c, OCPX_E_ARG, "ocpx_set_wells: well " + std::to_string(seen[k].second) +
                                            " perforates cell " + std::to_string(seen[k].first) + " twice");
-    void* old[13] = {c->wells, c->pcell, c->pWI, c->pdG, c->pxi, c->Bnw, c->Cwn, c->Dw, c->wres, c->qw,
-                     c->wrhs, c->dinv_w, c->wrow};
+    void* old[20] = {c->wells, c->pcell, c->pWI, c->pdG, c->pxi, c->Bnw, c->Cwn, c->Dw, c->wres, c->qw,
+                     c->wrhs, c->dinv_w, c->wrow, c->qperf, c->bdiag, c->dD, c->pwell, c->uptr, c->uperf, c->ucell};
     for (void* p : old) dfree(c, p);
     c->wells = nullptr; c->pcell = nullptr; c->pWI = c->pdG = c->pxi = nullptr;
     c->Bnw = c->Cwn = c->Dw = c->wres = c->qw = c->wrhs = c->dinv_w = nullptr;
+    c->qperf = c->bdiag = c->dD = nullptr;
+    c->pwell = c->uptr = c->uperf = nullptr;
+    c->ucell = nullptr;
+    c->nucell = 0;
     c->wrow = nullptr;
     c->nwell = 0; c->nperf = 0;
     c->wells_jac = false;
@@ -945,8 +967,32 @@ int ocpx_set_wells(ocpx_ctx* c, int32_t nwell, const ocpx_well* wells, const int
         (rc = dalloc(c, &c->Cwn, np_ * B)) || (rc = dalloc(c, &c->Dw, (size_t)nwell * B * B)) ||
         (rc = dalloc(c, &c->wres, nwell)) || (rc = dalloc(c, &c->qw, (size_t)nwell * c->nc)) ||
         (rc = dalloc(c, &c->wrhs, (size_t)nwell * B)) || (rc = dalloc(c, &c->dinv_w, (size_t)nwell * B * B)) ||
-        (rc = dalloc(c, &c->wrow, (size_t)nwell + 1)))
+        (rc = dalloc(c, &c->wrow, (size_t)nwell + 1)) || (rc = dalloc(c, &c->qperf, (size_t)np_ * c->nc)) ||
+        (rc = dalloc(c, &c->bdiag, (size_t)np_ * B * B)) || (rc = dalloc(c, &c->dD, (size_t)np_)) ||
+        (rc = dalloc(c, &c->pwell, (size_t)np_)) || (rc = dalloc(c, &c->uperf, (size_t)np_)))
         return rc;
+    {
+        // perforation -> well, and the perforated cells with their perforations in order
+        std::vector<int32_t> pw(np_);
+        for (int32_t w = 0; w < nwell; w++)
+            for (int32_t p = wells[w].perf0; p < wells[w].perf0 + wells[w].nperf; p++) pw[p] = w;
+        std::vector<std::pair<int64_t, int32_t>> byc(np_);
+        for (int64_t p = 0; p < np_; p++) byc[p] = {perf_cell[p], (int32_t)p};
+        std::sort(byc.begin(), byc.end());
+        std::vector<int64_t> uc;
+        std::vector<int32_t> up, upf(np_);
+        for (int64_t k = 0; k < np_; k++) {
+            if (k == 0 || byc[k].first != byc[k - 1].first) { uc.push_back(byc[k].first); up.push_back((int32_t)k); }
+            upf[k] = byc[k].second;
+        }
+        up.push_back((int32_t)np_);
+        if ((rc = dalloc(c, &c->ucell, uc.size())) || (rc = dalloc(c, &c->uptr, up.size()))) return rc;
+        HIPCHK(c, hipMemcpy(c->pwell, pw.data(), 4 * pw.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->uperf, upf.data(), 4 * upf.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->ucell, uc.data(), 8 * uc.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->uptr, up.data(), 4 * up.size(), hipMemcpyHostToDevice));
+        c->nucell = (int32_t)uc.size();
+    }
     {
         std::vector<int64_t> wr(nwell + 1);
         for (int32_t w = 0; w <= nwell; w++) wr[w] = w;

@@ -127,15 +127,23 @@ hipError_t run_norms_final(int64_t nblocks, const double* blockmax, const int64_
 hipError_t run_res_norms(int nc, const CellIO& c, const double* res, double* blockmax, int64_t* blockarg,
                          hipStream_t st);
 
-// Peaceman wells (wells.hip): one workgroup walks the wells and their
-// perforations in order (the reference's update sequence, no atomics)
+// Peaceman wells (wells.hip): per perforation in parallel, then the bulk
+// rows per perforated cell and the well rows per well, each in perforation
+// order (the reference's update sequence, no atomics)
 struct WellIO {
-    int32_t nwell;
+    int32_t nwell, nperf, nucell;
     const ocpx_well* wells;     // [nwell] device copy; bhp is the current iterate
     const int64_t* cell;        // [nperf] perforated cell
     const double* WI;           // [nperf] WI * multiplier (CONV_DARCY included)
     const double* dG;           // [nperf] perforation head
     const double* xi;           // [nperf] injected fluid's molar density (injectors)
+    const int32_t* pwell;       // [nperf] well of each perforation
+    const int64_t* ucell;       // [nucell] perforated cells, ascending
+    const int32_t* uptr;        // [nucell + 1] their perforations in uperf
+    const int32_t* uperf;       // [nperf] perforations grouped by cell, in perforation order
+    double* qperf;              // [nperf][nc] perforation component rates
+    double* bdiag;              // [nperf][b][b] perforation block added to the bulk diagonal
+    double* dD;                 // [nperf] the perforation's share of the well diagonal (0, 0)
     double* Bnw;                // [nperf][b] column 0 of block (n, w)
     double* Cwn;                // [nperf][b] row 0 of block (w, n)
     double* Dw;                 // [nwell][b][b]
