@@ -59,7 +59,47 @@ def parse():
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only copy a buffer of this many MiB on the GPU (calibrates the "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
+    p.add_argument("--wells", type=str, default="",
+                   help="only run the well workload on this grid (e.g. 100x100x50: five vertical wells "
+                        "through every layer, four Newton passes with the well terms, one solve) and exit")
     return p.parse_args()
+
+
+def wells_run(dims):
+    """Five vertical wells (BHP producers at the corners, a rate injector at
+    the centre) through every layer of the heterogeneous grid: four Newton
+    passes with the well terms and one device solve with the well rows.
+    The kernel times come from a kernel trace of this run."""
+    from opencaeporox_amd import wells as WL
+    from opencaeporox_amd import workload as W
+    from opencaeporox_amd.fim import BATCHED, IsoTFIM
+    nx, ny, nz = (int(v) for v in dims.split("x"))
+    wl = W.heterogeneous_grid(nx, ny, nz, Nc=6)
+    g, nc = wl.grid, wl.Nc + 1
+    col = lambda i, j: [i + j * g.nx + k * g.nx * g.ny for k in range(g.nz)]
+    P = wl.state["P"]
+    wells = []
+    for name, (i, j) in (("P1", (0, 0)), ("P2", (nx - 1, 0)), ("P3", (0, ny - 1)), ("P4", (nx - 1, ny - 1))):
+        c = col(i, j)
+        wells.append(WL.PeacemanWell(name, WL.PROD, WL.BHP, bhp=float(P[c].min()) - 10, target=float(P[c].min()) - 10,
+                                     cells=c, WI=[5.0] * nz, dG=list(0.07 * np.arange(nz))))
+    c = col(nx // 2, ny // 2)
+    zi = np.zeros(nc)
+    zi[0] = 1.0
+    wells.append(WL.PeacemanWell("I", WL.INJ, WL.RATE, bhp=float(P[c].max()) + 10, target=100.0, cells=c,
+                                 WI=[5.0] * nz, dG=list(0.05 * np.arange(nz)), xi=[8.0] * nz, zi=list(zi),
+                                 factor=[1.0] * nc))
+    dev = IsoTFIM.from_workload(wl)
+    dev.SetWells(wells)
+    for k in range(4):
+        dev.newton_pass(1.0, BATCHED)
+        dev.sync()
+        t = time.perf_counter()
+        dev.CalResWells(1.0, jacobian=True)
+        log(f"pass {k}: well terms {1e3 * (time.perf_counter() - t):.3f} ms (host clock, with the norm copies)")
+    u, it, rr = dev.LinearSolve(rtol=1e-3, maxit=50)
+    log(f"solve with {len(wells)} well rows: {it} iterations, ||r||/||b|| {rr:.3e}")
+    dev.close()
 
 
 def log(*a):
@@ -370,6 +410,10 @@ def main():
 
     def sync():
         torch.cuda.synchronize()
+
+    if a.wells:
+        wells_run(a.wells)
+        return
 
     if a.copy_mib:
         x = torch.ones(a.copy_mib << 17, dtype=torch.float64, device="cuda")
