@@ -223,6 +223,8 @@ class Deck:
     def _table(self, L, kw):
         # InputTABLE: rows of colNum values; '/' at a row's end closes a
         # table, a '/' line ends the keyword
+        if kw == "ZMFVD":
+            self._need_comps(kw)
         col = TABLE_COLUMNS.get(kw, self.num_com + 1)
         tabs, cur = [], [[] for _ in range(col)]
         while True:

@@ -255,14 +255,81 @@ def injector_heads(fluid, well, well_depth, perf_depth, T, water=False, maxlen=1
 
 
 def cell_state(dev, cells):
-    """The perforated cells' fields producer_heads reads, from the device."""
+    """The perforated cells' fields producer_heads and check_opt_mode read,
+    from the device."""
     nb, nc = dev.nb, dev.nc
     c = np.asarray(cells)
-    return {"P": dev.get("P")[c], "T": dev.get("T")[c],
+    return {"P": dev.get("P")[c], "T": dev.get("T")[c], "PJ": dev.get("PJ").reshape(nb, 3)[c],
             "PHASEEXIST": dev.get("PHASEEXIST").reshape(nb, 3)[c].astype(bool),
             "KR": dev.get("KR").reshape(nb, 3)[c], "MU": dev.get("MU").reshape(nb, 3)[c],
             "XI": dev.get("XI").reshape(nb, 3)[c], "XIJ": dev.get("XIJ").reshape(nb, 3, nc)[c],
             "NI": dev.get("NI").reshape(nb, nc)[c]}
+
+
+@dataclass
+class WellLimits:
+    """The control data CheckOptMode reads (WellOpt): the mode the schedule
+    asks for, the rate limit, the limiting BHP (maxBHP for injectors, minBHP
+    for producers), the BHP target, the producer's phase weights and the
+    injected fluid's temperature (degC) and phase."""
+    init_mode: int
+    max_rate: float
+    limit_bhp: float
+    tar_bhp: float
+    weights: tuple = (1.0, 0.0, 0.0)
+    inj_T: float = 20.0
+    water: bool = False
+
+
+def check_opt_mode(fluid, cell, well, lim, flash, Psurf=PSURF, Tsurf=TSURF):
+    """PeacemanWell::CheckOptMode (src/WellPeaceman.cpp:117-159) with
+    CalInjRateMaxBHP / CalProdRateMinBHP (:468-530): the well's rate at its
+    limiting BHP decides between its scheduled rate control and BHP control.
+    cell is the perforated cells' state (cell_state); the producer's stream
+    is flashed at (Psurf, Tsurf) with flash(P, T, Ni) (SegmentFlash).
+    Returns (mode, bhp, q): bhp becomes tarBHP under BHP control, else
+    stays; q is the rate at the limiting BHP (surface units)."""
+    npf = len(well.cells)
+    dG = np.asarray(well.dG if well.dG is not None else np.zeros(npf), float)
+    ex = cell["PHASEEXIST"]
+    tj = np.where(ex, np.asarray(well.WI)[:, None] * cell["KR"] / np.where(ex, cell["MU"], 1.0), 0.0)
+    Nc = fluid.nc
+    if well.type == INJ:
+        zi = np.asarray(well.zi, float)[:Nc]
+        q = 0.0
+        for p in range(npf):
+            Pperf = lim.limit_bhp + dG[p]
+            tinj = 0.0
+            for j in range(3):
+                if ex[p, j]:
+                    tinj += tj[p, j]
+            xi = xi_phase(fluid, Pperf, lim.inj_T, zi, water=lim.water)
+            q += tinj * xi * (Pperf - cell["P"][p])
+        vstd = 1.0 if lim.water else 1.0 / xi_phase(fluid, Psurf, Tsurf, zi)      # CalVmStd
+        q *= vstd
+    else:
+        nc = Nc + 1
+        qi = np.zeros(nc)
+        for p in range(npf):
+            Pperf = lim.limit_bhp + dG[p]
+            for j in range(3):
+                if ex[p, j]:
+                    temp = tj[p, j] * cell["XI"][p, j] * (cell["PJ"][p, j] - Pperf)
+                    qi += cell["XIJ"][p, j] * temp
+        if qi.sum() > 0:                                                         # CalVStd
+            exf, vj, _ = flash(Psurf, Tsurf, qi)
+            vj = np.where(exf, vj, 0.0)
+            w = np.asarray(lim.weights, float)
+            q = vj[0] * w[0] + vj[1] * w[1] + qi[nc - 1] * w[2]
+        else:
+            q = 0.0                                                              # no inflow at minBHP
+    if lim.init_mode == BHP:
+        if well.type == INJ and q > lim.max_rate:
+            return RATE, well.bhp, q
+        return BHP, lim.tar_bhp, q
+    if q > lim.max_rate:
+        return lim.init_mode, well.bhp, q
+    return BHP, lim.tar_bhp, q
 
 
 def full_system(rowptr, col, val, rhs, wells, Bnw, Cwn, Dw, wres, b):

@@ -254,3 +254,43 @@ def test_producer_heads_match_oracle(oracle):
     ok, worst = close(dG_d, dG_o)
     assert ok, f"worst {worst}"
     dev.close()
+
+
+def test_producer_mode_check_matches_oracle(oracle):
+    """CheckOptMode for a producer: the surface oil rate at minBHP from the
+    device flash against the same check over the oracle's states and flash;
+    the limit decides the control."""
+    from opencaeporox_amd.fim import BATCHED
+    wl, wells = _newton_case()
+    prod = wells[0]
+    nb, nc = wl.grid.nb, wl.Nc + 1
+    dev = IsoTFIM.from_workload(wl)
+    dev.CalFlash(BATCHED)
+    dev.CalKrPc()
+    st = state_from_workload(wl)
+    oracle.cal_flash(wl.fluid, st)
+    oracle.cal_krpc(wl.sat, st)
+    c = np.asarray(prod.cells)
+    cs = {"P": st.a["P"][c], "PJ": st.a["PJ"].reshape(nb, 3)[c],
+          "PHASEEXIST": st.a["PHASEEXIST"].reshape(nb, 3)[c].astype(bool), "KR": st.a["KR"].reshape(nb, 3)[c],
+          "MU": st.a["MU"].reshape(nb, 3)[c], "XI": st.a["XI"].reshape(nb, 3)[c],
+          "XIJ": st.a["XIJ"].reshape(nb, 3, nc)[c]}
+
+    def oflash(P, T, Ni):
+        w2 = WL.surface_workload(wl.fluid, np.asarray(Ni, float)[None, :], P, T)
+        s2 = state_from_workload(w2)
+        oracle.cal_flash(w2.fluid, s2)
+        return s2.a["PHASEEXIST"].astype(bool), s2.a["S"] * s2.a["VF"][0], s2.a["RHO"]
+
+    lim = WL.WellLimits(init_mode=WL.RATE, max_rate=0.0, limit_bhp=float(cs["P"].min()) - 20.0,
+                        tar_bhp=float(cs["P"].min()) - 20.0, weights=(1.0, 0.0, 0.0))
+    seg = WL.SegmentFlash(wl.fluid)
+    md, bd, qd = WL.check_opt_mode(wl.fluid, WL.cell_state(dev, prod.cells), prod, lim, seg)
+    mo, bo, qo = WL.check_opt_mode(wl.fluid, cs, prod, lim, oflash)
+    print("producer rate at minBHP:", qd, qo)
+    assert qd > 0 and (md, bd) == (mo, bo) == (WL.RATE, prod.bhp)
+    assert close(np.array([qd]), np.array([qo]))[0]
+    lim.max_rate = 2 * qd
+    assert WL.check_opt_mode(wl.fluid, WL.cell_state(dev, prod.cells), prod, lim, seg)[:2] == (WL.BHP, lim.tar_bhp)
+    seg.close()
+    dev.close()

@@ -208,3 +208,30 @@ def test_injector_heads_integrate_the_fluid_column():
         # each perforation integrates upwards from its own pressure (bhp + old dG = 200 bar)
         top = solve_ivp(ode, (d, 2000.0 if p == 0 else depths[p - 1]), [200.0], rtol=1e-11, atol=1e-9).y[0, -1]
         assert 200.0 - top == pytest.approx(np.diff(np.r_[0.0, fine])[p], rel=2e-5)
+
+
+def test_injector_mode_check(oracle):
+    """CheckOptMode for an injector: the rate at maxBHP is sum_p transINJ *
+    xi(Pperf) * (Pperf - P) times the injected fluid's surface molar volume
+    (here from the oracle's EoS), and it decides between rate and BHP control."""
+    fl = W.make_fluid(4, 101)
+    zi = [0.9, 0.06, 0.03, 0.01, 0.0]
+    cell = {"P": np.array([150.0, 152.0]), "PHASEEXIST": np.array([[1, 0, 1], [1, 1, 1]], bool),
+            "KR": np.array([[0.6, 0.0, 0.1], [0.3, 0.2, 0.05]]), "MU": np.array([[0.8, 0.0, 0.5], [0.9, 0.02, 0.5]])}
+    w = WL.PeacemanWell("I", WL.INJ, WL.RATE, bhp=170.0, target=50.0, cells=[0, 1], WI=[3.0, 4.0], dG=[0.0, 0.7],
+                        zi=zi)
+    lim = WL.WellLimits(init_mode=WL.RATE, max_rate=0.0, limit_bhp=180.0, tar_bhp=180.0, inj_T=60.0)
+    mode, bhp, q = WL.check_opt_mode(fl, cell, w, lim, flash=None)
+    expect = 0.0
+    for p in range(2):
+        Pp = 180.0 + w.dG[p]
+        tinj = sum(w.WI[p] * cell["KR"][p, j] / cell["MU"][p, j] for j in range(3) if cell["PHASEEXIST"][p, j])
+        expect += tinj / oracle.vmder(fl, Pp, 60.0 + W.CONV4, zi[:4])[0] * (Pp - cell["P"][p])
+    expect *= oracle.vmder(fl, WL.PSURF, WL.TSURF + W.CONV4, zi[:4])[0]
+    assert q == pytest.approx(expect, rel=1e-12)
+    assert (mode, bhp) == (WL.RATE, 170.0)                     # the potential exceeds the limit
+    lim.max_rate = 1.01 * q
+    assert WL.check_opt_mode(fl, cell, w, lim, flash=None)[:2] == (WL.BHP, 180.0)
+    lim.init_mode = WL.BHP
+    lim.max_rate = 0.99 * q
+    assert WL.check_opt_mode(fl, cell, w, lim, flash=None)[:2] == (WL.RATE, 170.0)
