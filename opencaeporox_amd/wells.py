@@ -332,6 +332,33 @@ def check_opt_mode(fluid, cell, well, lim, flash, Psurf=PSURF, Tsurf=TSURF):
     return BHP, lim.tar_bhp, q
 
 
+def check_cross_flow(cell_P, well, perf_open):
+    """PeacemanWell::CheckCrossFlow (src/WellPeaceman.cpp:573-652): the first
+    open perforation whose pressure difference points the wrong way (bulk
+    below the perforation pressure for a producer, above it for an
+    injector) is closed, closed perforations that point the right way again
+    reopen, and the deepest perforation opens if none is left.  Returns
+    (open flags, crossflow); a closed perforation has multiplier 0, so the
+    WI passed to ocpx_set_wells is WI * open."""
+    npf = len(well.cells)
+    dG = np.asarray(well.dG if well.dG is not None else np.zeros(npf), float)
+    perfP = well.bhp + dG
+    op = list(bool(v) for v in perf_open)
+    cross = False
+    for p in range(npf):
+        wrong = cell_P[p] < perfP[p] if well.type == PROD else cell_P[p] > perfP[p]
+        right = cell_P[p] > perfP[p] if well.type == PROD else cell_P[p] < perfP[p]
+        if op[p] and wrong:
+            op[p] = False
+            cross = True
+            break
+        if not op[p] and right:
+            op[p] = True
+    if not any(op):
+        op[-1] = True
+    return op, cross
+
+
 def full_system(rowptr, col, val, rhs, wells, Bnw, Cwn, Dw, wres, b):
     """The reference's OCPMatrix and right-hand side with the well rows
     (scalar CSR of size (nb + nwell) * b).  Well w is block row nb + w; a

@@ -235,3 +235,23 @@ def test_injector_mode_check(oracle):
     lim.init_mode = WL.BHP
     lim.max_rate = 0.99 * q
     assert WL.check_opt_mode(fl, cell, w, lim, flash=None)[:2] == (WL.RATE, 170.0)
+
+
+def test_cross_flow_check():
+    """CheckCrossFlow: the first reversed open perforation closes (and stops
+    the scan), closed ones pointing the right way reopen, and the deepest
+    opens when none is left."""
+    prod = WL.PeacemanWell("P", WL.PROD, WL.BHP, bhp=100.0, target=100.0, cells=[0, 1, 2, 3], WI=[1.0] * 4,
+                           dG=[0.0, 1.0, 2.0, 3.0])
+    # perforation pressures 100, 101, 102, 103
+    op, cross = WL.check_cross_flow(np.array([105.0, 100.5, 101.0, 110.0]), prod, [True, True, True, True])
+    assert cross and op == [True, False, True, True]        # perforation 1 reversed; the scan stops there
+    op, cross = WL.check_cross_flow(np.array([105.0, 102.0, 101.0, 110.0]), prod, [True, False, True, True])
+    assert cross and op == [True, True, False, True]        # 1 reopens, then 2 (reversed) closes
+    op, cross = WL.check_cross_flow(np.array([105.0, 102.0, 103.0, 110.0]), prod, [True, True, False, True])
+    assert not cross and op == [True, True, True, True]
+    inj = WL.PeacemanWell("I", WL.INJ, WL.BHP, bhp=100.0, target=100.0, cells=[0, 1], WI=[1.0] * 2, dG=[0.0, 1.0])
+    op, cross = WL.check_cross_flow(np.array([120.0, 130.0]), inj, [False, False])
+    assert not cross and op == [False, True]                # none open: the deepest opens
+    op, cross = WL.check_cross_flow(np.array([99.0, 130.0]), inj, [True, True])
+    assert cross and op == [True, False]
