@@ -106,6 +106,24 @@ def rate_factors(fluid, qi, weights, Psurf=PSURF, Tsurf=TSURF):
     return factors_from_flash(S, vf, qi, weights)
 
 
+def factor_stream(qw, cell, well):
+    """The moles CalFactor flashes for a producer (src/WellPeaceman.cpp:
+    665-688): the well's component rates qw [nc] when they are all
+    non-negative and sum above 1e-8, otherwise sum_p sum_j transj * xi_j *
+    x_ij over the open phases of the perforated cells (cell_state)."""
+    qw = np.asarray(qw, float)
+    if qw.sum() > 1e-8 and np.all(qw >= 0):
+        return qw.copy()
+    ex = cell["PHASEEXIST"]
+    tj = np.where(ex, np.asarray(well.WI)[:, None] * cell["KR"] / np.where(ex, cell["MU"], 1.0), 0.0)
+    out = np.zeros_like(qw)
+    for p in range(len(well.cells)):
+        for j in range(3):
+            if ex[p, j]:
+                out += tj[p, j] * cell["XI"][p, j] * cell["XIJ"][p, j]
+    return out
+
+
 def surface_workload(fluid, qi, Psurf=PSURF, Tsurf=TSURF):
     """One cell per well holding the produced moles at (Psurf, Tsurf)."""
     from . import workload as W

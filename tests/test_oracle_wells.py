@@ -255,3 +255,18 @@ def test_cross_flow_check():
     assert not cross and op == [False, True]                # none open: the deepest opens
     op, cross = WL.check_cross_flow(np.array([99.0, 130.0]), inj, [True, True])
     assert cross and op == [True, False]
+
+
+def test_factor_stream_fallback():
+    """CalFactor flashes the well's rates, or the perforation mobility stream
+    when a rate is negative or the total is not positive."""
+    nc = 3
+    cell = {"PHASEEXIST": np.array([[1, 1, 0]], bool), "KR": np.array([[0.5, 0.2, 0.0]]),
+            "MU": np.array([[1.0, 0.05, 0.0]]), "XI": np.array([[8.0, 2.0, 55.0]]),
+            "XIJ": np.array([[[0.2, 0.8, 0.0], [0.9, 0.1, 0.0], [0.0, 0.0, 1.0]]])}
+    w = WL.PeacemanWell("P", WL.PROD, WL.RATE, bhp=100.0, target=10.0, cells=[0], WI=[2.0])
+    q = np.array([1.0, 2.0, 0.5])
+    assert np.array_equal(WL.factor_stream(q, cell, w), q)
+    fb = 2.0 * 0.5 / 1.0 * 8.0 * np.array([0.2, 0.8, 0.0]) + 2.0 * 0.2 / 0.05 * 2.0 * np.array([0.9, 0.1, 0.0])
+    assert np.allclose(WL.factor_stream(np.array([1.0, -0.1, 0.5]), cell, w), fb, rtol=1e-15)
+    assert np.allclose(WL.factor_stream(np.zeros(nc), cell, w), fb, rtol=1e-15)
