@@ -171,6 +171,9 @@ class Deck:
     arrays: dict = field(default_factory=dict)         # grid keyword -> np.ndarray
     welspecs: list = field(default_factory=list)
     compdat: list = field(default_factory=list)
+    schedule: list = field(default_factory=list)       # (step, keyword, record) of WCONPROD / WCONINJE
+    wellstre: dict = field(default_factory=dict)       # injected stream name -> composition
+    step: int = 0                                      # TSTEP keywords read so far
     skipped: list = field(default_factory=list)        # keywords not read
 
     # ------------------------------------------------------------ parsing
@@ -544,6 +547,86 @@ class Deck:
             raise DeckError(f"well {name} has no COMPDAT perforation")
         return out
 
+    def well_controls(self, name, step=0):
+        """The WCONPROD / WCONINJE record of a well in force at a schedule step
+        (fuzzy names ending in '*' match by prefix, as the reference)."""
+        rec = None
+        for st, kw, r in self.schedule:
+            if st > step:
+                break
+            pat = r[0]
+            star = pat.find("*")
+            if name == pat or (star >= 0 and name[:star] == pat[:star]):
+                rec = (kw, r)
+        return rec
+
+    def to_wells(self, step=0):
+        """(PeacemanWell, WellLimits) for every well with a control at the
+        step, in METRIC: WellOpt (src/WellOpt.cpp:14-70) -- producers ORAT /
+        GRAT / WRAT / LRAT (rate control, phase weights) or BHP with the
+        minimum BHP as target; injectors of water or a WELLSTRE stream, RATE
+        or BHP with the maximum BHP as target.  FIELD rates are stb/day
+        (oil, water) or Mscf/day (gas), converted to m3/day.  bhp starts at
+        the target BHP; dG is zero and the rate weights are left to
+        wells.rate_factors."""
+        from . import wells as WL
+        fi = self.field_units
+        stb, mscf = (0.158987294928, 28.316846592) if fi else (1.0, 1.0)
+        pconv = PSI if fi else 1.0
+        weights = {"ORAT": (1.0, 0.0, 0.0), "GRAT": (0.0, 1.0, 0.0), "WRAT": (0.0, 0.0, 1.0),
+                   "LRAT": (1.0, 0.0, 1.0)}
+        nc = self.num_com + 1
+        out = []
+        names = []
+        for w in self.welspecs:
+            if w[0] not in names:
+                names.append(w[0])
+        for name in names:
+            ctl = self.well_controls(name, step)
+            if ctl is None:
+                continue
+            kw, r = ctl
+            cells, wi = zip(*self.well_perforations(name))
+            if kw == "WCONPROD":
+                state, mode = r[1], r[2]
+                if state != "OPEN":
+                    continue
+                rate = 1e10 if r[3] == "DEFAULT" else _float(r[3], kw)
+                minbhp = _float(r[4], kw) * pconv
+                if mode == "BHP":
+                    lim = WL.WellLimits(WL.BHP, rate, minbhp, minbhp)
+                    target = minbhp
+                else:
+                    wgt = weights[mode]
+                    rate *= mscf if mode == "GRAT" else stb
+                    lim = WL.WellLimits(WL.RATE, rate, minbhp, minbhp, weights=wgt)
+                    target = rate
+                well = WL.PeacemanWell(name, WL.PROD, lim.init_mode, bhp=minbhp, target=target, cells=list(cells),
+                                       WI=list(wi), dG=[0.0] * len(cells))
+            else:
+                fluid, state, mode = r[1].upper(), r[2], r[3]
+                if state != "OPEN":
+                    continue
+                water = fluid in ("WAT", "WATER")
+                zi = [0.0] * nc
+                if water:
+                    zi[nc - 1] = 1.0
+                else:
+                    comp = {k.upper(): v for k, v in self.wellstre.items()}.get(fluid)
+                    if comp is None:
+                        raise DeckError(f"{name}: injected fluid {r[1]} is neither water nor a WELLSTRE stream")
+                    for i, v in enumerate(comp[:nc - 1]):
+                        zi[i] = v
+                rate = 1e10 if r[4] == "DEFAULT" else _float(r[4], kw) * (stb if water else mscf)
+                maxbhp = _float(r[5], kw) * pconv
+                init = WL.BHP if mode == "BHP" else WL.RATE
+                lim = WL.WellLimits(init, rate, maxbhp, maxbhp, inj_T=self.temperature_c(), water=water)
+                well = WL.PeacemanWell(name, WL.INJ, init, bhp=maxbhp, target=maxbhp if init == WL.BHP else rate,
+                                       cells=list(cells), WI=list(wi), dG=[0.0] * len(cells), zi=zi,
+                                       factor=[1.0] * nc)
+            out.append((well, lim))
+        return out
+
     def composition(self, depth_m, region=0):
         """Hydrocarbon mole fractions per cell from ZMFVD (depth-interpolated,
         constant outside the table, normalised)."""
@@ -711,6 +794,31 @@ def _records(attr):
     return h
 
 
+def _controls(d, L, t):
+    # WellOptParam (src/ParamWell.cpp:14-47): the records apply from the current schedule step
+    while True:
+        v = L.need(t[0])
+        if v[0] == "/":
+            break
+        d.schedule.append((d.step, t[0], deal_default(v)))
+
+
+def _wellstre(d, L, t):
+    while True:
+        v = L.need("WELLSTRE")
+        if v[0] == "/":
+            break
+        d.wellstre[v[0]] = [_float(x, "WELLSTRE") for x in v[1:] if x != "/"]
+
+
+def _tstep(d, L, t):
+    while True:
+        v = L.need("TSTEP")
+        if v[0] == "/" or v[-1] == "/":
+            break
+    d.step += 1
+
+
 _HANDLERS = {
     "FIELD": _units("FIELD"), "METRIC": _units("METRIC"),
     "COMPS": _comps, "NCOMPS": _comps, "CNAMES": _cnames, "BIC": _bic, "LBCCOEF": _lbccoef,
@@ -721,6 +829,7 @@ _HANDLERS = {
     "MULTIPLY": lambda d, L, t: d._equals(L, "MULTIPLY", "mul"),
     "COPY": lambda d, L, t: d._copy(L, "COPY"),
     "WELSPECS": _records("welspecs"), "COMPDAT": _records("compdat"),
+    "WCONPROD": _controls, "WCONINJE": _controls, "WELLSTRE": _wellstre, "TSTEP": _tstep,
 }
 
 

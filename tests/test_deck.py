@@ -260,3 +260,23 @@ def test_include_and_metric_deck(tmp_path):
     assert d.arrays["PERMZ"].tolist() == [200, 200, 300, 400]
     g = d.to_grid()
     assert np.allclose(g.depth, [100.5, 100.5, 102.0, 102.0])
+
+
+def test_schedule_controls(spe5, spe3):
+    """WCONPROD / WCONINJE / WELLSTRE by schedule step, converted to METRIC."""
+    from opencaeporox_amd import wells as WL
+    w0 = spe5.to_wells(step=0)
+    assert [w.name for w, _ in w0] == ["PROD1"]
+    prod, lim = w0[0]
+    assert prod.type == WL.PROD and prod.mode == WL.RATE and lim.weights == (1.0, 0.0, 0.0)
+    assert prod.target == pytest.approx(12000 * 0.158987294928) and lim.limit_bhp == pytest.approx(1000 * deck.PSI)
+    assert prod.cells == [146]
+    w2 = dict((w.name, (w, l)) for w, l in spe5.to_wells(step=2))
+    inj, lim = w2["I"]
+    assert inj.type == WL.INJ and not lim.water and inj.zi[:3] == [0.77, 0.2, 0.03]
+    assert inj.target == pytest.approx(12000 * 28.316846592) and lim.tar_bhp == pytest.approx(10000 * deck.PSI)
+    w1 = dict((w.name, (w, l)) for w, l in spe5.to_wells(step=1))
+    assert w1["I"][1].water and w1["I"][0].zi[-1] == 1.0
+    w = dict((w.name, (w, l)) for w, l in spe3.to_wells(step=0))
+    assert w["PROD"][1].weights == (0.0, 1.0, 0.0) and w["PROD"][0].cells == [222, 303]
+    assert w["INJ"][0].target == pytest.approx(4700 * 28.316846592)

@@ -80,22 +80,29 @@ def test_wells_match_oracle(oracle, mode_prod):
 
 
 def test_spe5_deck_wells(oracle):
+    """The SPE5 deck's own wells at its third schedule step (WELSPECS /
+    COMPDAT / WCONPROD / WCONINJE / WELLSTRE): the ORAT producer with rate
+    weights from the device surface flash of its mobility stream, the
+    solvent injector with its molar density from ocpx_xi_phase."""
     d = deck.read_deck(os.path.join(DECKS, "spe5", "spe5.data"))
     _, pd = d.datum()
     rng = np.random.Generator(np.random.PCG64(3))
     wl = d.to_workload(seed=3, P=pd * rng.uniform(0.3, 1.1, d.nb))
     nc = wl.Nc + 1
-    (pc, pwi), = d.well_perforations("PROD1")
-    (ic, iwi), = d.well_perforations("I")
-    P = wl.state["P"]
-    zi = np.zeros(nc)
-    zi[:3] = [0.77, 0.20, 0.03]                       # the deck's WELLSTRE solvent
-    wells = [WL.PeacemanWell("PROD1", WL.PROD, WL.RATE, bhp=float(P[pc]) - 20.0, target=12000 * 0.159,
-                             cells=[pc], WI=[pwi], factor=list(np.full(nc, 0.12))),
-             WL.PeacemanWell("I", WL.INJ, WL.BHP, bhp=float(P[ic]) + 30.0, target=float(P[ic]) + 30.0,
-                             cells=[ic], WI=[iwi], zi=list(zi),
-                             xi=[WL.xi_phase(wl.fluid, float(P[ic]) + 30.0, d.temperature_c(), zi[:nc - 1])])]
+    wells = [w for w, _ in d.to_wells(step=2)]
+    assert [w.name for w in wells] == ["PROD1", "I"]
     dev = IsoTFIM.from_workload(wl)
+    st = state_from_workload(wl)
+    oracle.cal_flash(wl.fluid, st)
+    _upload_flash(dev, st)
+    dev.CalKrPc()
+    prod, inj = wells
+    stream = WL.factor_stream(np.zeros(nc), WL.cell_state(dev, prod.cells), prod)
+    prod.factor = list(np.full(nc, WL.rate_factors(wl.fluid, stream[None, :], np.array([[1.0, 0.0, 0.0]]))[0]))
+    P = wl.state["P"]
+    prod.bhp = float(P[prod.cells[0]]) - 20.0
+    inj.bhp = float(P[inj.cells[0]]) + 30.0
+    inj.xi = [WL.xi_phase(wl.fluid, inj.bhp, d.temperature_c(), inj.zi[:nc - 1])]
     _check(dev, oracle, wl, wells, "spe5")
     dev.close()
 
