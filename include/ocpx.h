@@ -273,9 +273,10 @@ int  ocpx_get_flash_queue_order(ocpx_ctx* ctx, int32_t* longest_first);
 int  ocpx_krpc_der(ocpx_ctx* ctx);
 /* ≙ IsothermalMethod::CalRock (src/IsoThermalMethod.cpp:19-30) */
 int  ocpx_cal_rock(ocpx_ctx* ctx);
-/* ≙ IsoT_FIM::CalRes (src/IsoThermalMethod.cpp:1090-1185) without wells.
- * Keeps resAbs (negated, [(nb)*(nc+1)]) on the device; norms and resAbs
- * are copied to the host when the pointers are non-NULL (synchronous then). */
+/* ≙ IsoT_FIM::CalRes (src/IsoThermalMethod.cpp:1090-1185), bulk part (the
+ * wells' part is ocpx_well_terms).  Keeps resAbs (negated, [(nb)*(nc+1)])
+ * on the device; norms and resAbs are copied to the host when the pointers
+ * are non-NULL (synchronous then). */
 int  ocpx_cal_res(ocpx_ctx* ctx, double dt, double* resAbs, ocpx_resnorms* norms);
 /* ≙ IsoT_FIM::AssembleMatBulks + LinearSystem::CopyRhs
  * (src/IsoThermalMethod.cpp:1187-1269, 742-752) into the device BSR. */
