@@ -301,3 +301,59 @@ def test_producer_mode_check_matches_oracle(oracle):
     assert WL.check_opt_mode(wl.fluid, WL.cell_state(dev, prod.cells), prod, lim, seg)[:2] == (WL.BHP, lim.tar_bhp)
     seg.close()
     dev.close()
+
+
+def test_spe5_time_step_with_its_producer(oracle):
+    """One SPE5 time step driven by the deck: the ORAT producer's schedule
+    control, CheckOptMode at minBHP (device surface flash), the rate weights
+    from the mobility stream, then Newton iterations with the well row on
+    the device (assemble -> well terms -> solve -> update).  The same step
+    on the oracle with a direct solve converges to bhp 268.6245 bar with the
+    rate on its 12000 stb/day target."""
+    import scipy.sparse.linalg as spla
+    from opencaeporox_amd.fim import BATCHED
+    d = deck.read_deck(os.path.join(DECKS, "spe5", "spe5.data"))
+    wl = d.to_workload(seed=3)
+    (prod, lim), = d.to_wells(step=0)
+    nb, b, nc = wl.grid.nb, wl.Nc + 2, wl.Nc + 1
+    dev = IsoTFIM.from_workload(wl)
+    dev.newton_pass(DT, BATCHED)
+    cs = WL.cell_state(dev, prod.cells)
+    seg = WL.SegmentFlash(wl.fluid)
+    mode, _, q = WL.check_opt_mode(wl.fluid, cs, prod, lim, seg)
+    seg.close()
+    assert mode == WL.RATE and q > prod.target
+    prod.mode = mode
+    prod.bhp = float(cs["P"][0]) - 5.0
+    stream = WL.factor_stream(np.zeros(nc), cs, prod)
+    prod.factor = list(np.full(nc, WL.rate_factors(wl.fluid, stream[None, :], np.array([lim.weights]))[0]))
+    # the oracle's step with a direct solve
+    ow = WL.PeacemanWell(**{**prod.__dict__})
+    st = state_from_workload(wl)
+    for _ in range(6):
+        oracle.cal_flash(wl.fluid, st); oracle.cal_krpc(wl.sat, st); oracle.cal_rock(wl.rock, st)
+        res, _ = oracle.cal_res(wl.fluid, st, wl.grid, DT)
+        A = oracle.assemble(wl.fluid, st, wl.grid, DT)
+        _, wres, _, Bo, Co, Do = oracle.wells(wl.fluid, st, DT, [ow], res, A)
+        J, f = WL.full_system(A[0], A[1], A[2], res, [ow], Bo, Co, Do, wres, b)
+        u = spla.spsolve(J.tocsc(), f)
+        oracle.get_solution(wl.fluid, st, u[:nb * b], 0.2)
+        ow.bhp += u[nb * b]
+    # the device step
+    dev.SetWells([prod])
+    bulk, wrow = [], []
+    for k in range(6):
+        dev.newton_pass(DT, BATCHED)
+        _, w = dev.CalResWells(DT, jacobian=True)
+        bulk.append(np.abs(dev.bsr()[3]).max())
+        wrow.append(abs(w[0]))
+        u, it, rr = dev.LinearSolve(rtol=1e-12, maxit=2000)
+        assert it > 0
+        dev.GetSolution(None, 0.2)
+    bhp, qw = dev.wells_state()
+    print("max |R|:", " ".join(f"{v:.2e}" for v in bulk), "| well row:", " ".join(f"{v:.2e}" for v in wrow),
+          f"| bhp {bhp[0]:.7f} (oracle {ow.bhp:.7f})")
+    assert bulk[-1] < 1e-9 * bulk[0] and wrow[-1] < 1e-8 * wrow[0]
+    assert bhp[0] == pytest.approx(ow.bhp, rel=1e-9)
+    assert qw[0] @ np.array(prod.factor) == pytest.approx(prod.target, rel=1e-8)
+    dev.close()
