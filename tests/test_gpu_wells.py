@@ -357,3 +357,68 @@ def test_spe5_time_step_with_its_producer(oracle):
     assert bhp[0] == pytest.approx(ow.bhp, rel=1e-9)
     assert qw[0] @ np.array(prod.factor) == pytest.approx(prod.target, rel=1e-8)
     dev.close()
+
+
+def test_spe3_time_step_with_its_wells(oracle):
+    """One SPE3 time step driven by the deck (gas condensate near its dew
+    point, 272 of 324 cells two-phase): the solvent injector and the GRAT
+    producer under rate control after CheckOptMode, the producer's weights
+    from its mobility stream, the injector's from the stream's surface molar
+    volume; device Newton iterations with both well rows against the
+    oracle's step with a direct solve.  The intermediate iterates pass
+    through rounding-sensitive flash cells, so the converged state is
+    compared."""
+    import scipy.sparse.linalg as spla
+    from opencaeporox_amd.fim import BATCHED
+    d = deck.read_deck(os.path.join(DECKS, "spe3", "spe3.data"))
+    wl = d.to_workload(seed=3)
+    nb, b, nc = wl.grid.nb, wl.Nc + 2, wl.Nc + 1
+    dev = IsoTFIM.from_workload(wl)
+    dev.newton_pass(DT, BATCHED)
+    seg = WL.SegmentFlash(wl.fluid)
+    wells = []
+    for w, lim in d.to_wells(step=0):
+        cs = WL.cell_state(dev, w.cells)
+        w.mode, _, q = WL.check_opt_mode(wl.fluid, cs, w, lim, seg)
+        assert w.mode == WL.RATE and q > w.target
+        if w.type == WL.PROD:
+            w.bhp = float(cs["P"].min()) - 5.0
+            stream = WL.factor_stream(np.zeros(nc), cs, w)
+            w.factor = list(np.full(nc, WL.rate_factors(wl.fluid, stream[None, :], np.array([lim.weights]))[0]))
+        else:
+            zhc = np.array(w.zi[:nc - 1])
+            w.bhp = float(cs["P"].max()) + 5.0
+            w.xi = [WL.xi_phase(wl.fluid, w.bhp, d.temperature_c(), zhc)] * len(w.cells)
+            w.factor = [1.0 / WL.xi_phase(wl.fluid, WL.PSURF, WL.TSURF, zhc)] * nc
+        wells.append(w)
+    seg.close()
+    ows = [WL.PeacemanWell(**{**w.__dict__}) for w in wells]
+    st = state_from_workload(wl)
+    for _ in range(8):
+        oracle.cal_flash(wl.fluid, st); oracle.cal_krpc(wl.sat, st); oracle.cal_rock(wl.rock, st)
+        res, _ = oracle.cal_res(wl.fluid, st, wl.grid, DT)
+        A = oracle.assemble(wl.fluid, st, wl.grid, DT)
+        _, wres, _, Bo, Co, Do = oracle.wells(wl.fluid, st, DT, ows, res, A)
+        J, f = WL.full_system(A[0], A[1], A[2], res, ows, Bo, Co, Do, wres, b)
+        u = spla.spsolve(J.tocsc(), f)
+        oracle.get_solution(wl.fluid, st, u[:nb * b], 0.2)
+        for k, w in enumerate(ows):
+            w.bhp += u[(nb + k) * b]
+    dev.SetWells(wells)
+    bulk = []
+    for k in range(8):
+        dev.newton_pass(DT, BATCHED)
+        _, wres = dev.CalResWells(DT, jacobian=True)
+        bulk.append(np.abs(dev.bsr()[3]).max())
+        u, it, rr = dev.LinearSolve(rtol=1e-12, maxit=3000)
+        assert it > 0
+        dev.GetSolution(None, 0.2)
+    bhp, qw = dev.wells_state()
+    print("max |R|:", " ".join(f"{v:.2e}" for v in bulk), "| bhp", bhp, "oracle", [w.bhp for w in ows])
+    assert bulk[-1] < 1e-9 * bulk[0] and np.abs(wres).max() < 1e-6
+    assert np.allclose(bhp, [w.bhp for w in ows], rtol=1e-8)
+    for k, w in enumerate(wells):
+        rate = qw[k] @ np.array(w.factor)
+        assert abs(rate) == pytest.approx(w.target, rel=1e-8)
+    assert np.allclose(dev.get("P"), st.a["P"], rtol=1e-9)
+    dev.close()
