@@ -310,7 +310,10 @@ int  ocpx_set_wells(ocpx_ctx* ctx, int32_t nwell, const ocpx_well* wells, const 
  * (PeacemanWell::CalTrans + CalFlux, PeacemanWellIsoT::CalResFIM and
  * AssembleMatInjFIM / AssembleMatProdFIM, src/WellPeaceman.cpp:365-462,
  * 1226-1612) on the current state.  Call it once after ocpx_cal_res
- * (jacobian = 0) or ocpx_assemble (jacobian = 1) with the same dt.  It adds
+ * (jacobian = 0) or ocpx_assemble (jacobian = 1) with the same dt; a second
+ * call on the same residual, or a call after the state changed (flash,
+ * ocpx_get_solution, ocpx_set_field, reset, geometry), returns OCPX_E_STATE
+ * instead of adding the wells twice or to a stale system.  It adds
  * the perforation rates to the bulk residual and right-hand side and
  * recomputes the norms (norms may be NULL); with jacobian = 1 it adds the
  * perforation terms to the bulk diagonal blocks and forms the well blocks
@@ -355,8 +358,11 @@ double ocpx_xi_phase(const ocpx_fluid* fluid, double P, double T, const double* 
  * ||r|| / ||b|| < rtol (the reference's stop_type 1; its decks use 1e-3 and
  * 200 iterations, data/spe5/bsr.fasp).  *iters = iterations, negative when
  * not converged (the reference's convention); *relres = final ||r||/||b||.
- * u (host [nb*(nc+1)], may be NULL) receives the solution, which also stays
- * on the device for ocpx_get_solution(ctx, NULL, dSmax).  When
+ * u (host [nb*(nc+1)], may be NULL) receives the last iterate; only a
+ * converged solution stays on the device for ocpx_get_solution(ctx, NULL,
+ * dSmax), which returns OCPX_E_STATE after a solve that did not converge
+ * (the reference never applies such a solution: it cuts dt and resets,
+ * src/IsoThermalMethod.cpp:776-781).  When
  * ocpx_well_terms(jacobian = 1) ran after the last residual / assembly pass
  * the system has the well rows too (AddDim, src/WellPeaceman.cpp:1285): u is
  * then [(nb + nwell)*(nc+1)], and ocpx_get_solution(ctx, NULL, dSmax) also

@@ -99,6 +99,8 @@ struct ocpx_ctx {
     int32_t nucell = 0;
     int64_t* wrow = nullptr;     // [nwell + 1] 0, 1, ..: block-row pointer of the well diagonal blocks
     bool wells_jac = false;      // the assembled system holds the well rows (ocpx_well_terms, jacobian = 1)
+    bool wells_applied = false;  // the current residual already holds the well terms
+    bool res_current = false;    // residual / Jacobian were formed from the current state
     int32_t solved_wells = 0;    // well rows in the last ocpx_linear_solve
     int64_t sol_cap = 0;         // length of the solver vectors
     // timing
@@ -368,6 +370,8 @@ int launch_krpc_rock(ocpx_ctx* c)
 int launch_res_asm(ocpx_ctx* c, double dt, bool dr, bool dj)
 {
     c->wells_jac = false;              // a new residual / Jacobian holds no well terms yet
+    c->wells_applied = false;
+    c->res_current = true;
     {
         TimerScope t(c, dj ? 4 : 3);       // the residual / assembly kernel alone
         HIPCHK(c, run_res_asm(c->nc, cell_io(c), conn_io(c), bsr_io(c), dt, dr, dj, c->resAbs, c->bmax, c->barg,
@@ -685,6 +689,7 @@ int ocpx_set_field(ocpx_ctx* c, int field, const void* host)
     }
     if (field == OCPX_F_DSEC_DPRI) c->flashed = true;   // dXsdXp supplied by the caller
     c->aux_valid = false;
+    c->res_current = false;
     if (field == OCPX_F_FTYPE_IN) {
         const int32_t* fi = (const int32_t*)host;
         bool any = false;
@@ -853,6 +858,7 @@ int ocpx_flash_der(ocpx_ctx* c, int mode)
     if (rc) return rc;
     c->flashed = true;
     c->aux_valid = false;
+    c->res_current = false;
     return OCPX_OK;
 }
 
@@ -1013,7 +1019,12 @@ int ocpx_well_terms(ocpx_ctx* c, double dt, int32_t jacobian, ocpx_resnorms* nor
     if (!c) return OCPX_E_ARG;
     if (c->nwell == 0) return fail(c, OCPX_E_STATE, "ocpx_well_terms: no wells (ocpx_set_wells)");
     if (jacobian && !c->assembled) return fail(c, OCPX_E_STATE, "ocpx_well_terms: the Jacobian needs ocpx_assemble first");
+    if (!c->res_current)
+        return fail(c, OCPX_E_STATE, "ocpx_well_terms: the state changed since the last ocpx_cal_res / ocpx_assemble");
+    if (c->wells_applied)
+        return fail(c, OCPX_E_STATE, "ocpx_well_terms: the wells are already in this residual / Jacobian");
     HIPCHK(c, run_well_terms(c->nc, cell_io(c), bsr_io(c), well_io(c), dt, jacobian != 0, c->resAbs, c->st));
+    c->wells_applied = true;
     if (jacobian) c->wells_jac = true;
     HIPCHK(c, run_res_norms(c->nc, cell_io(c), c->resAbs, c->bmax, c->barg, c->st));
     HIPCHK(c, run_norms_final(res_asm_blocks(c->nc, c->nb), c->bmax, c->barg, c->norms4, c->st));
@@ -1184,6 +1195,7 @@ int ocpx_get_solution(ocpx_ctx* c, const double* u, double dSmax)
         HIPCHK(c, run_well_bhp_update(c->wells, c->solved_wells, (int)B, c->nb, c->sol, c->st));
     c->aux_valid = false;
     c->solved = false;
+    c->res_current = false;
     return OCPX_OK;
 }
 
@@ -1289,8 +1301,10 @@ int ocpx_linear_solve(ocpx_ctx* c, double rtol, int32_t maxit, double* u, int32_
         HIPCHK(c, hipMemcpyAsync(u, x, (size_t)len * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));
     }
-    c->solved = true;
-    c->solved_wells = nwj;
+    // a solve that did not converge leaves nothing to apply (the reference cuts dt and
+    // resets instead, src/IsoThermalMethod.cpp:776-781)
+    c->solved = conv;
+    c->solved_wells = conv ? nwj : 0;
     return OCPX_OK;
 }
 
@@ -1336,6 +1350,7 @@ int ocpx_reset_to_last_step(ocpx_ctx* c)
     if (!c) return OCPX_E_ARG;
     if (!c->have_last) return fail(c, OCPX_E_STATE, "ocpx_reset_to_last_step before ocpx_update_last_step");
     c->aux_valid = false;
+    c->res_current = false;
     for (int fid : kLastStepFields)
         HIPCHK(c, hipMemcpyAsync(c->f[fid].d, c->lf[fid].d, field_bytes(c, fid), hipMemcpyDeviceToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->f[OCPX_F_NI].d, c->f[OCPX_F_LNI].d, field_bytes(c, OCPX_F_NI),
@@ -1350,6 +1365,7 @@ int ocpx_set_geometry(ocpx_ctx* c, const double* v, const double* trans)
     if (v) HIPCHK(c, hipMemcpy(c->v, v, (size_t)c->nb * 8, hipMemcpyHostToDevice));
     if (trans && c->nconn) HIPCHK(c, hipMemcpy(c->trans, trans, (size_t)c->nconn * 8, hipMemcpyHostToDevice));
     c->aux_valid = false;
+    c->res_current = false;
     return OCPX_OK;
 }
 
@@ -1389,6 +1405,8 @@ int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
     HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
     c->flashed = c->assembled = true;
     c->wells_jac = false;
+    c->wells_applied = false;
+    c->res_current = true;
     return OCPX_OK;
 }
 

@@ -548,15 +548,18 @@ class Deck:
         return out
 
     def well_controls(self, name, step=0):
-        """The WCONPROD / WCONINJE record of a well in force at a schedule step
-        (fuzzy names ending in '*' match by prefix, as the reference)."""
+        """The WCONPROD / WCONINJE record of a well in force at a schedule step,
+        the step counting TSTEP values as the reference's criticalTime does.  A
+        name with '*' matches every well whose name contains the text before the
+        '*' (ParamWell::InputWCONPROD / InputWCONINJE, src/ParamWell.cpp:222-286:
+        name.find(src))."""
         rec = None
         for st, kw, r in self.schedule:
             if st > step:
                 break
             pat = r[0]
             star = pat.find("*")
-            if name == pat or (star >= 0 and name[:star] == pat[:star]):
+            if name == pat or (star >= 0 and pat[:star] in name):
                 rec = (kw, r)
         return rec
 
@@ -812,11 +815,16 @@ def _wellstre(d, L, t):
 
 
 def _tstep(d, L, t):
+    # ParamWell::InputTSTEP (src/ParamWell.cpp:288-307): every time value adds one
+    # critical time, and the controls read next apply from index criticalTime.size() - 1
     while True:
         v = L.need("TSTEP")
-        if v[0] == "/" or v[-1] == "/":
+        if v[0] == "/":
             break
-    d.step += 1
+        v = deal_default(v)
+        d.step += sum(1 for x in v if x != "/")
+        if v[-1] == "/":
+            break
 
 
 _HANDLERS = {

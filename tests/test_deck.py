@@ -271,12 +271,16 @@ def test_schedule_controls(spe5, spe3):
     assert prod.type == WL.PROD and prod.mode == WL.RATE and lim.weights == (1.0, 0.0, 0.0)
     assert prod.target == pytest.approx(12000 * 0.158987294928) and lim.limit_bhp == pytest.approx(1000 * deck.PSI)
     assert prod.cells == [146]
+    # the first TSTEP holds two values (2*365.25), so the injector's controls start at
+    # criticalTime index 2: none at step 1, WATER at step 2, Solvent at step 3
+    assert [w.name for w, _ in spe5.to_wells(step=1)] == ["PROD1"]
     w2 = dict((w.name, (w, l)) for w, l in spe5.to_wells(step=2))
-    inj, lim = w2["I"]
+    assert w2["I"][1].water and w2["I"][0].zi[-1] == 1.0
+    w3 = dict((w.name, (w, l)) for w, l in spe5.to_wells(step=3))
+    inj, lim = w3["I"]
     assert inj.type == WL.INJ and not lim.water and inj.zi[:3] == [0.77, 0.2, 0.03]
     assert inj.target == pytest.approx(12000 * 28.316846592) and lim.tar_bhp == pytest.approx(10000 * deck.PSI)
-    w1 = dict((w.name, (w, l)) for w, l in spe5.to_wells(step=1))
-    assert w1["I"][1].water and w1["I"][0].zi[-1] == 1.0
+    assert spe5.step == 20          # 19 TSTEP keywords: one of 2*365.25, eighteen of 1*365.25
     w = dict((w.name, (w, l)) for w, l in spe3.to_wells(step=0))
     assert w["PROD"][1].weights == (0.0, 1.0, 0.0) and w["PROD"][0].cells == [222, 303]
     assert w["INJ"][0].target == pytest.approx(4700 * 28.316846592)

@@ -56,6 +56,19 @@ def test_nonconvergence_is_reported():
     dev.newton_pass(1.0, BATCHED)
     u, it, rr = dev.LinearSolve(rtol=1e-30, maxit=3)
     assert it == -3 and rr > 1e-30
+    # a solve that did not converge leaves nothing to apply: the state is unchanged
+    P0 = dev.get("P").copy()
+    from opencaeporox_amd.fim import OcpxError
+    with pytest.raises(OcpxError, match="needs ocpx_linear_solve"):
+        dev.GetSolution(None)
+    assert np.array_equal(dev.get("P"), P0)
+    # after a converged solve it applies once
+    u, it, rr = dev.LinearSolve(rtol=1e-1, maxit=200)
+    assert it > 0
+    dev.GetSolution(None)
+    assert not np.array_equal(dev.get("P"), P0)
+    with pytest.raises(OcpxError, match="needs ocpx_linear_solve"):
+        dev.GetSolution(None)
     dev.close()
 
 

@@ -80,7 +80,7 @@ def test_wells_match_oracle(oracle, mode_prod):
 
 
 def test_spe5_deck_wells(oracle):
-    """The SPE5 deck's own wells at its third schedule step (WELSPECS /
+    """The SPE5 deck's own wells at schedule step 3 (criticalTime index) (WELSPECS /
     COMPDAT / WCONPROD / WCONINJE / WELLSTRE): the ORAT producer with rate
     weights from the device surface flash of its mobility stream, the
     solvent injector with its molar density from ocpx_xi_phase."""
@@ -89,7 +89,7 @@ def test_spe5_deck_wells(oracle):
     rng = np.random.Generator(np.random.PCG64(3))
     wl = d.to_workload(seed=3, P=pd * rng.uniform(0.3, 1.1, d.nb))
     nc = wl.Nc + 1
-    wells = [w for w, _ in d.to_wells(step=2)]
+    wells = [w for w, _ in d.to_wells(step=3)]
     assert [w.name for w in wells] == ["PROD1", "I"]
     dev = IsoTFIM.from_workload(wl)
     st = state_from_workload(wl)
@@ -119,6 +119,31 @@ def test_set_wells_rejects_bad_input():
         dev.SetWells([bad])
     with pytest.raises(OcpxError, match="no wells"):
         dev.CalResWells(DT)
+    dev.close()
+
+
+def test_well_terms_once_per_residual():
+    """ocpx_well_terms adds the wells to the residual and Jacobian in place: a
+    second call on the same residual, or a call after the state moved, is an
+    error instead of a double count."""
+    from opencaeporox_amd.fim import BATCHED, OcpxError
+    wl, wells = _newton_case()
+    dev = IsoTFIM.from_workload(wl)
+    dev.SetWells(wells)
+    dev.newton_pass(DT, BATCHED)
+    _, w1 = dev.CalResWells(DT, jacobian=False)
+    with pytest.raises(OcpxError, match="already"):
+        dev.CalResWells(DT, jacobian=True)
+    dev.AssembleMatBulks(DT)                      # a new residual + Jacobian takes them again
+    _, w2 = dev.CalResWells(DT, jacobian=True)
+    assert np.array_equal(w1, w2)
+    with pytest.raises(OcpxError, match="already"):
+        dev.CalResWells(DT, jacobian=False)
+    u, it, _ = dev.LinearSolve(rtol=1e-6, maxit=500)
+    assert it > 0
+    dev.GetSolution(None)
+    with pytest.raises(OcpxError, match="state changed"):
+        dev.CalResWells(DT, jacobian=False)
     dev.close()
 
 
