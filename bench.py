@@ -75,20 +75,7 @@ def wells_run(dims):
     from opencaeporox_amd.fim import BATCHED, IsoTFIM
     nx, ny, nz = (int(v) for v in dims.split("x"))
     wl = W.heterogeneous_grid(nx, ny, nz, Nc=6)
-    g, nc = wl.grid, wl.Nc + 1
-    col = lambda i, j: [i + j * g.nx + k * g.nx * g.ny for k in range(g.nz)]
-    P = wl.state["P"]
-    wells = []
-    for name, (i, j) in (("P1", (0, 0)), ("P2", (nx - 1, 0)), ("P3", (0, ny - 1)), ("P4", (nx - 1, ny - 1))):
-        c = col(i, j)
-        wells.append(WL.PeacemanWell(name, WL.PROD, WL.BHP, bhp=float(P[c].min()) - 10, target=float(P[c].min()) - 10,
-                                     cells=c, WI=[5.0] * nz, dG=list(0.07 * np.arange(nz))))
-    c = col(nx // 2, ny // 2)
-    zi = np.zeros(nc)
-    zi[0] = 1.0
-    wells.append(WL.PeacemanWell("I", WL.INJ, WL.RATE, bhp=float(P[c].max()) + 10, target=100.0, cells=c,
-                                 WI=[5.0] * nz, dG=list(0.05 * np.arange(nz)), xi=[8.0] * nz, zi=list(zi),
-                                 factor=[1.0] * nc))
+    wells = WL.corner_and_centre_wells(wl)
     dev = IsoTFIM.from_workload(wl)
     dev.SetWells(wells)
     for k in range(4):

@@ -32,8 +32,20 @@ def flags():
             "-Wno-unused-result", "-I", os.path.join(PKG, "..", "include")]
 
 
-def compile_one(src):
+def _stale(src, obj):
+    """The object is older than its source or any header of this directory."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(HERE, src)] + [os.path.join(HERE, h) for h in os.listdir(HERE) if h.endswith(".hpp")]
+    deps.append(os.path.join(PKG, "..", "include", "ocpx.h"))
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def compile_one(src, force=True):
     obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+    if not force and not _stale(src, obj):
+        return obj
     cmd = [hipcc()] + flags() + ["-c", os.path.join(HERE, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -41,11 +53,12 @@ def compile_one(src):
     return obj
 
 
-def build(verbose=False):
+def build(verbose=False, force=True):
+    """Compile every source (force) or only the out-of-date ones, then link."""
     os.makedirs(OBJ_DIR, exist_ok=True)
     workers = min(len(SOURCES), int(os.environ.get("MAX_JOBS", "4")))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
+        objs = list(ex.map(lambda s: compile_one(s, force), SOURCES))
     out = os.path.join(OUT_DIR, "libocpx.so")
     tmp = out + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
@@ -59,5 +72,5 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, force="--changed" not in sys.argv)
     sys.exit(0)

@@ -407,3 +407,26 @@ def full_system(rowptr, col, val, rhs, wells, Bnw, Cwn, Dw, wres, b):
     for w in range(nw):
         f[(nb + w) * b] = -wres[w]
     return A, f
+
+
+def corner_and_centre_wells(wl, WI=5.0):
+    """The five vertical wells of the heterogeneous-grid workload, through
+    every layer: BHP producers at the four corners 10 bar below the lowest
+    pressure they perforate, and a rate injector of the first component at
+    the centre (molar density 8, 100 units per day), with hydrostatic-like
+    perforation heads."""
+    g, nc = wl.grid, wl.Nc + 1
+    nx, ny, nz = g.nx, g.ny, g.nz
+    col = lambda i, j: [i + j * nx + k * nx * ny for k in range(nz)]
+    P = wl.state["P"]
+    out = []
+    for name, (i, j) in (("P1", (0, 0)), ("P2", (nx - 1, 0)), ("P3", (0, ny - 1)), ("P4", (nx - 1, ny - 1))):
+        c = col(i, j)
+        out.append(PeacemanWell(name, PROD, BHP, bhp=float(P[c].min()) - 10, target=float(P[c].min()) - 10,
+                                cells=c, WI=[WI] * nz, dG=list(0.07 * np.arange(nz))))
+    c = col(nx // 2, ny // 2)
+    zi = np.zeros(nc)
+    zi[0] = 1.0
+    out.append(PeacemanWell("I", INJ, RATE, bhp=float(P[c].max()) + 10, target=100.0, cells=c, WI=[WI] * nz,
+                            dG=list(0.05 * np.arange(nz)), xi=[8.0] * nz, zi=list(zi), factor=[1.0] * nc))
+    return out

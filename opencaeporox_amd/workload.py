@@ -278,7 +278,7 @@ def cube_grid(seed=1):
     return Workload("10x10x10 cubes, Nc=6", fl, make_sat(), rock, grid, st)
 
 
-def heterogeneous_grid(nx=100, ny=100, nz=50, Nc=6, seed=3):
+def heterogeneous_grid(nx=100, ny=100, nz=50, Nc=6, seed=3, lbc_exact=0):
     """Hydrostatic P + N(0, 1 MPa), ln kx ~ N(ln 100, 2), kz = 0.1 kx."""
     rng = np.random.Generator(np.random.PCG64(seed))
     nb = nx * ny * nz
@@ -286,6 +286,7 @@ def heterogeneous_grid(nx=100, ny=100, nz=50, Nc=6, seed=3):
     poro = rng.uniform(0.1, 0.3, nb)
     grid = cartesian_grid(nx, ny, nz, 6.096, kx, kx, 0.1 * kx, poro)
     fl = make_fluid(Nc, seed + 100)
+    fl.lbc_exact = lbc_exact
     rock = make_rock()
     P = 200.0 + 0.0980665 * (grid.depth - grid.depth.min()) + rng.normal(0, 10.0, nb)
     z = rng.dirichlet(np.ones(Nc), nb)

@@ -13,6 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "build", "libocpx_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libocpref.so")
+REF_FLASH_SO = os.path.join(HERE, "_ref", "libocpref_flash.so")
+MKL_RT = "/opt/conda/lib/libmkl_rt.so.1"
 
 D = np.float64
 
@@ -350,3 +352,42 @@ class Ref:
         self.lib.ref_eos_deck(units.encode(), n, *[_p(v) for v in a[:6]], len(a[5]), P, T, _p(a[6]),
                               _p(phi), _p(abz), _p(vm))
         return phi, abz, vm[0]
+
+
+class RefFlash:
+    """The reference's own compositional flash (oracle/_ref/libocpref_flash.so):
+    OCPPhaseEquilibrium, OCPMixtureMethodComp01::FlashDer and SkipPSA, with
+    the LAPACK/BLAS calls of src/DenseMat.cpp served by the image's MKL
+    runtime (sequential, so every run gives the same bits)."""
+
+    def __init__(self, path=REF_FLASH_SO):
+        os.environ.setdefault("MKL_THREADING_LAYER", "SEQUENTIAL")
+        os.environ.setdefault("MKL_CBWR", "COMPATIBLE")
+        self._mkl = C.CDLL(MKL_RT, mode=C.RTLD_GLOBAL)
+        self.lib = L = C.CDLL(path)
+        vp, i64 = C.c_void_p, C.c_int64
+        L.ref_phase_equilibrium.argtypes = [vp, i64] + [vp] * 11
+        L.ref_cal_flash.argtypes = [vp, vp]
+
+    @staticmethod
+    def available():
+        return os.path.exists(REF_FLASH_SO) and os.path.exists(MKL_RT)
+
+    def phase_equilibrium(self, fl, P, T, z, ftype=None, lNP=None, lx=None):
+        """As Oracle.phase_equilibrium; the last output holds the per-cell
+        SSMSTA, NRSTA, SSMSP, NRSP and RR iteration counts of the reference."""
+        n, Nc = z.shape
+        P, T, z = (np.ascontiguousarray(a, D) for a in (P, T, z))
+        NP = np.zeros(n, np.int32); nu = np.zeros(2 * n); x = np.zeros(2 * n * Nc)
+        fo = np.zeros(n, np.int32); it = np.zeros(5 * n, np.int64)
+        ft = None if ftype is None else np.ascontiguousarray(ftype, np.int32)
+        ln = None if lNP is None else np.ascontiguousarray(lNP, np.int32)
+        lxx = None if lx is None else np.ascontiguousarray(lx, D)
+        self.lib.ref_phase_equilibrium(C.byref(fl), n, _p(P), _p(T), _p(z), _p(ft), _p(ln), _p(lxx),
+                                       _p(NP), _p(nu), _p(x), _p(fo), _p(it))
+        return NP, nu.reshape(n, 2), x.reshape(n, 2, Nc), fo, it.reshape(n, 5)
+
+    def cal_flash(self, fl, st):
+        """IsoT_FIM::CalFlash + PassFlashValue on an OracleState (ITERS and
+        the ill markers are left as they are)."""
+        b = st.struct(); self.lib.ref_cal_flash(C.byref(fl), C.byref(b))

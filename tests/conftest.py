@@ -38,17 +38,19 @@ def ref():
 
 
 def close(a, b, rtol=1e-10, scale=None):
-    """|a-b| <= rtol*(|b| + scale); scale defaults to 1e-2*max|b| (entries far
-    below the field's magnitude are held to an absolute bound)."""
+    """|a-b| <= rtol*(|b| + scale); scale defaults to FLOOR*max|b| (entries
+    more than 1/FLOOR below the field's largest are held to rtol*FLOOR of it,
+    see oracle/parity.py)."""
+    from oracle.parity import FLOOR
     a = np.asarray(a, float)
     b = np.asarray(b, float)
     if scale is None:
-        scale = 1e-2 * (np.max(np.abs(b)) if b.size else 0.0)
+        scale = FLOOR * (np.max(np.abs(b)) if b.size else 0.0)
     err = np.abs(a - b)
     bound = rtol * (np.abs(b) + scale)
     ok = (err <= bound) | (a == b)
     return bool(np.all(ok)), float(np.max(err / np.maximum(np.abs(b) + scale, 1e-300))) if b.size else 0.0
 
 
-from oracle.parity import (PARITY_RTOL, SPLIT_ILL, SPREAD_MULT, perturbed_oracle_states,  # noqa: E402,F401
-                           rel_err, rounding_sensitive, rounding_spread)
+from oracle.parity import (FLOOR, PARITY_RTOL, SPLIT_ILL, SPREAD_MULT, label_stable,  # noqa: E402,F401
+                           perturbed_oracle_states, rel_err, rounding_sensitive, rounding_spread)

@@ -6,8 +6,8 @@ import pytest
 from opencaeporox_amd import workload as W
 from opencaeporox_amd.fim import BATCHED, PLAIN, IsoTFIM
 from oracle.oracle import state_from_workload
-from tests.conftest import (PARITY_RTOL, SPREAD_MULT, close, perturbed_oracle_states, rel_err,
-                            rounding_sensitive, rounding_spread)
+from tests.conftest import (FLOOR, PARITY_RTOL, SPREAD_MULT, close, label_stable, perturbed_oracle_states,
+                            rel_err, rounding_spread)
 
 pytestmark = pytest.mark.gpu
 
@@ -20,49 +20,59 @@ def _bits(v):
     return "|".join(str(1 << b) for b in range(10) if (v >> b) & 1) or "0"
 
 
-def _compare_flash(dev, st, Nc, tag, others, max_mismatch=0.02):
-    """Labels identical on every cell whose reference trajectory is not
-    rounding-sensitive (no split / Rachford-Rice ill bit in the oracle); all
-    fields within PARITY_RTOL on those cells, or within SPREAD_MULT times the
-    oracle's own spread under rounding-level perturbations (others) where
-    that is larger.  Elsewhere the reference's own outcome depends on
-    rounding: label mismatches there are counted, must stay rare, and are
-    listed with their oracle bits."""
+def _compare_flash(dev, st, Nc, tag, others, min_cells=0, min_two_phase=0):
+    """Labels identical on every cell whose labels the oracle reproduces under
+    rounding-level perturbations (others: its FMA build, P * (1 + 2^-50)),
+    capped trajectories included.  Every field within PARITY_RTOL on every
+    cell where device and oracle labels agree, or within SPREAD_MULT times the
+    oracle's own spread under those perturbations where that is larger.  Prints
+    per field how many entries pass only because of the scale floor (FLOOR)
+    and asserts the compared-cell counts.  Returns the label-agreeing cells."""
     n = dev.nb
     pe_d = dev.get("PHASEEXIST").reshape(n, 3)
     pe_o = st.a["PHASEEXIST"].reshape(n, 3)
     mism = np.any(pe_d != pe_o, axis=1)
-    sens = rounding_sensitive(st.ill)
-    bad = np.where(mism & ~sens)[0]
-    print(f"{tag}: {mism.sum()} label mismatches of {n} ({sens.sum()} rounding-sensitive cells); "
-          f"oracle bits of mismatches: {[_bits(int(st.ill[c])) for c in np.where(mism)[0][:20]]}")
-    assert bad.size == 0, (f"{tag}: labels differ in {bad.size} cells that are not rounding-sensitive, "
+    stable = label_stable(st, others)
+    bad = np.where(mism & stable)[0]
+    print(f"{tag}: {stable.sum()} of {n} cells label-stable in the oracle; {mism.sum()} label mismatches, "
+          f"all among the {(~stable).sum()} unstable cells; oracle bits of mismatches: "
+          f"{[_bits(int(st.ill[c])) for c in np.where(mism)[0][:20]]}")
+    assert bad.size == 0, (f"{tag}: labels differ in {bad.size} label-stable cells, "
                            f"e.g. {bad[:10]} with bits {[_bits(int(st.ill[c])) for c in bad[:10]]}")
-    assert mism.mean() <= max_mismatch, f"{tag}: {mism.sum()} label mismatches"
-    keep = ~mism & ~sens
+    same = ~mism
     fo_d, fo_o = dev.get("FTYPE_OUT"), st.a["FTYPE_OUT"]
-    assert np.array_equal(fo_d[keep], fo_o[keep]), f"{tag}: ftype differs"
+    assert np.array_equal(fo_d[same & stable], fo_o[same & stable]), f"{tag}: ftype differs"
     loose = np.zeros(n, bool)
     for f in FLASH_FIELDS + ["DSEC_DPRI"]:
         rowwise = f == "DSEC_DPRI"
         b = st.a[f].reshape(n, -1)
         a = dev.get(f).reshape(n, -1)
         err = np.zeros_like(b)
-        err[keep] = rel_err(a[keep], b[keep], rowwise)
+        err0 = np.zeros_like(b)
+        err[same] = rel_err(a[same], b[same], rowwise)
+        err0[same] = rel_err(a[same], b[same], rowwise, floor=0.0)
         spread = rounding_spread(st, others, f, n, rowwise)
+        spread0 = rounding_spread(st, others, f, n, rowwise, floor=0.0)
         tol = np.maximum(PARITY_RTOL, SPREAD_MULT * spread)
+        tol0 = np.maximum(PARITY_RTOL, SPREAD_MULT * spread0)
         over = np.any(err > PARITY_RTOL, axis=1)
         loose |= over
         fail = np.any(err > tol, axis=1)
-        if over.any():
+        floor_only = int(np.sum((err0 > tol0) & (err <= tol)))
+        if over.any() or floor_only:
             c = np.where(over)[0]
             print(f"{tag}: {f} above {PARITY_RTOL:g} in {c.size} cells (worst {err.max():.2e}; oracle's own "
-                  f"rounding spread there up to {spread[c].max():.2e}), e.g. {c[:6]}")
+                  f"rounding spread there up to {spread[c].max() if c.size else 0:.2e}); {floor_only} entries "
+                  f"pass only by the {FLOOR:g} scale floor (worst plain relative {err0.max():.2e})")
         assert not fail.any(), (f"{tag}: {f} differs beyond tolerance in cells {np.where(fail)[0][:8]}: "
                                 f"err {err[fail].max():.2e}, spread {spread[fail].max():.2e}")
-    print(f"{tag}: {keep.sum() - loose[keep].sum()} of {keep.sum()} compared cells within {PARITY_RTOL:g}, "
-          f"{loose[keep].sum()} within {SPREAD_MULT:g}x the oracle's rounding spread")
-    return ~mism
+    two = same & (pe_o[:, 0] > 0) & (pe_o[:, 1] > 0)
+    print(f"{tag}: {same.sum()} cells field-compared ({two.sum()} two-phase): "
+          f"{same.sum() - loose[same].sum()} within {PARITY_RTOL:g}, "
+          f"{loose[same].sum()} within {SPREAD_MULT:g}x the oracle's rounding spread")
+    assert same.sum() >= min_cells, f"{tag}: only {same.sum()} cells compared"
+    assert two.sum() >= min_two_phase, f"{tag}: only {two.sum()} two-phase cells compared"
+    return same & stable
 
 
 @pytest.mark.parametrize("mode,lanes", [(PLAIN, 1), (BATCHED, 1), (BATCHED, 4), (BATCHED, 8)])
@@ -74,9 +84,11 @@ def test_flash_config1(oracle, oracle_fma, mode, lanes):
     others = perturbed_oracle_states(oracle, oracle_fma, wl.fluid, st)
     dev.CalFlash(mode)
     oracle.cal_flash(wl.fluid, st)
-    same = _compare_flash(dev, st, wl.Nc, f"10x10x10 mode {mode} lanes {lanes}", others)
+    # the 202 two-phase cells of this grid all stop Rachford-Rice at its cap in
+    # the reference algorithm; they are compared like the others
+    keep = _compare_flash(dev, st, wl.Nc, f"10x10x10 mode {mode} lanes {lanes}", others,
+                          min_cells=980, min_two_phase=200)
     # SkipPSA next-step data
-    keep = same & ~rounding_sensitive(st.ill)
     assert np.array_equal(dev.get("SKIP_FLAG")[keep], st.a["SKIP_FLAG"][keep])
     ok, worst = close(dev.get("SKIP_MINEIG")[keep], st.a["SKIP_MINEIG"][keep], rtol=1e-6)
     assert ok, f"minEig worst {worst}"
@@ -104,31 +116,42 @@ def test_phase_equilibrium_cells(oracle, oracle_fma, Nc, mode, lanes):
     T = wl.state["T"] + W.CONV4
     NPo, nuo, xo, fo, ito = oracle.phase_equilibrium(wl.fluid, P, T, z)
     ill = oracle.last_ill
-    sens = rounding_sensitive(ill)
-    mism = NP != NPo
-    print(f"Nc={Nc}: {mism.sum()} NP mismatches of {len(P)}, rounding-sensitive cells {sens.sum()}")
-    bad = np.where(mism & ~sens)[0]
-    assert bad.size == 0, f"Nc={Nc}: NP differs in {bad.size} cells {bad[:10]}"
-    assert mism.mean() < 0.02
-    two = (NP == 2) & (NPo == 2) & ~sens
-    assert two.sum() > 100, "workload must contain well-conditioned two-phase cells"
     runs = [oracle_fma.phase_equilibrium(wl.fluid, P, T, z),
             oracle.phase_equilibrium(wl.fluid, P * (1 + 2.0 ** -50), T, z)]
+    # NP must agree wherever the oracle reproduces its own NP under the
+    # rounding-level perturbations, capped trajectories included
+    stable = np.ones(len(P), bool)
+    for r in runs:
+        stable &= r[0] == NPo
+    mism = NP != NPo
+    print(f"Nc={Nc}: {stable.sum()} of {len(P)} cells NP-stable in the oracle, {mism.sum()} NP mismatches")
+    bad = np.where(mism & stable)[0]
+    assert bad.size == 0, f"Nc={Nc}: NP differs in {bad.size} NP-stable cells {bad[:10]}"
+    assert mism.mean() < 0.02
+    two = (NP == 2) & (NPo == 2)
+    assert (two & (ill == 0)).sum() > 100, "workload must contain well-conditioned two-phase cells"
     for name, dv, ov, k in (("nu", nu, nuo, 1), ("x", x, xo, 2)):
         dv, ov = dv[two].reshape(two.sum(), -1), ov[two].reshape(two.sum(), -1)
         err = rel_err(dv, ov)
+        err0 = rel_err(dv, ov, floor=0.0)
         spread = np.zeros_like(err)
+        spread0 = np.zeros_like(err)
         for r in runs:
             same = (r[0] == NPo)[two]
             e = rel_err(r[k][two].reshape(two.sum(), -1), ov)
+            e0 = rel_err(r[k][two].reshape(two.sum(), -1), ov, floor=0.0)
             e[~same] = 0
+            e0[~same] = 0
             spread = np.maximum(spread, e)
+            spread0 = np.maximum(spread0, e0)
         tol = np.maximum(PARITY_RTOL, SPREAD_MULT * spread)
+        tol0 = np.maximum(PARITY_RTOL, SPREAD_MULT * spread0)
         over = np.any(err > PARITY_RTOL, axis=1)
         print(f"Nc={Nc}: {name} above {PARITY_RTOL:g} in {over.sum()} of {two.sum()} two-phase cells "
-              f"(worst {err.max():.2e}, oracle spread there {spread[over].max() if over.any() else 0:.2e})")
+              f"(worst {err.max():.2e}, oracle spread there {spread[over].max() if over.any() else 0:.2e}); "
+              f"{int(np.sum((err0 > tol0) & (err <= tol)))} entries pass only by the scale floor")
         assert np.all(err <= tol), f"{name}: err {err.max():.2e}"
-    assert np.array_equal(dev.get("FTYPE_OUT")[~sens], fo[~sens])
+    assert np.array_equal(dev.get("FTYPE_OUT")[stable], fo[stable])
     # iteration counts agree on clean cells (same algorithm; a converged
     # fixed-point iteration may take one more or one fewer step under
     # different rounding)
@@ -183,8 +206,15 @@ def _compare_assembly(dev, oracle, wl, st, dt, tag):
     assert ok
     rp_d, col_d, val_d, rhs_d = dev.bsr()
     assert np.array_equal(rp_d, rp) and np.array_equal(col_d, col)
-    scale = 1e-2 * np.max(np.abs(val), axis=(1, 2), keepdims=True)
+    # each entry relative to itself, floored at FLOOR times the largest entry
+    # of its row of the block (one equation's derivatives in one block)
+    scale = FLOOR * np.max(np.abs(val), axis=2, keepdims=True)
     err = np.abs(val_d - val) / (np.abs(val) + scale + 1e-300)
+    err[val_d == val] = 0
+    err0 = np.abs(val_d - val) / (np.abs(val) + 1e-300)
+    err0[val_d == val] = 0
+    print(f"{tag}: BSR worst {err.max():.2e}; {int(np.sum((err0 > 1e-10) & (err <= 1e-10)))} of {val.size} "
+          f"entries pass only by the scale floor (worst plain relative {err0.max():.2e})")
     assert np.max(err) <= 1e-10, (f"{tag}: BSR worst {np.max(err):.3e} at block "
                                   f"{np.unravel_index(np.argmax(err), err.shape)}")
     ok, worst = close(rhs_d, res_o)
@@ -229,7 +259,7 @@ def test_newton_passes_end_to_end(oracle, oracle_fma, mode, lanes):
     dev.CalFlash(mode)
     oracle.cal_flash(wl.fluid, st)
     assert st.ill.max() == 0
-    _compare_flash(dev, st, wl.Nc, "pass1", others, max_mismatch=0)
+    _compare_flash(dev, st, wl.Nc, "pass1", others, min_cells=dev.nb)
     _compare_assembly(dev, oracle, wl, st, dt, "pass1")
     rng = np.random.Generator(np.random.PCG64(5))
     u = rng.normal(0, 1, dev.nb * dev.bs)
@@ -243,7 +273,7 @@ def test_newton_passes_end_to_end(oracle, oracle_fma, mode, lanes):
     others = perturbed_oracle_states(oracle, oracle_fma, wl.fluid, st)
     dev.CalFlash(mode)
     oracle.cal_flash(wl.fluid, st)
-    _compare_flash(dev, st, wl.Nc, "pass2", others, max_mismatch=0)
+    _compare_flash(dev, st, wl.Nc, "pass2", others, min_cells=dev.nb)
     _compare_assembly(dev, oracle, wl, st, dt, "pass2")
 
 
