@@ -261,6 +261,17 @@ int  ocpx_get_flash_mapping(ocpx_ctx* ctx, int32_t* lanes_per_cell);
  * (0 where one does not apply), then the chosen one in natural order.
  * Synchronous. */
 int  ocpx_autotune_flash(ocpx_ctx* ctx, double* ms);
+/* Stages of the batched flash.  No counterpart in the reference; same
+ * results either way.  1 (default): in a context of one model, the strict
+ * stability test (StableSSM + StableNR, src/OCPPhaseEquilibrium.cpp:397-521)
+ * runs in its own stages -- the SSM loops of every cell's trial compositions
+ * in a lean kernel, the Newton steps of the trials that need one in a
+ * second, then per cell the first unstable trial in the reference's order --
+ * and the mapped kernel of ocpx_set_flash_mapping takes only the cells left
+ * (splits, flash types 1 and 2, the relaxed test); 0: the mapped kernel takes
+ * every cell.  Contexts of several models always run unstaged. */
+int  ocpx_set_flash_stages(ocpx_ctx* ctx, int on);
+int  ocpx_get_flash_stages(ocpx_ctx* ctx, int32_t* on);
 /* Queue order of the batched flash.  No counterpart in the reference (its
  * flash loops over cells in order); same results either way.  1 (default):
  * each batched flash hands out the cells longest first, by their fugacity-

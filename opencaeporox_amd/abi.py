@@ -132,6 +132,8 @@ def load(path=None):
         "ocpx_linear_solve": (C.c_int, [vp, dbl, i32, vp, C.POINTER(i32), C.POINTER(dbl)]),
         "ocpx_set_flash_queue_order": (C.c_int, [vp, C.c_int]),
         "ocpx_get_flash_queue_order": (C.c_int, [vp, C.POINTER(i32)]),
+        "ocpx_set_flash_stages": (C.c_int, [vp, C.c_int]),
+        "ocpx_get_flash_stages": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_timing_enable": (C.c_int, [vp, C.c_int]),
         "ocpx_timing_get": (C.c_int, [vp, C.c_int, C.POINTER(dbl), C.POINTER(i64)]),
         "ocpx_set_wells": (C.c_int, [vp, i32, vp, vp, vp, vp, vp]),

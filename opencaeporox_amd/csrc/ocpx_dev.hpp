@@ -39,6 +39,7 @@ struct DevFluid {
     double lbc[5];               // LBC coefficients x10
     double auxB[OCPX_NCMAX], sqrtMW[OCPX_NCMAX];
     double pvtw[5], rhoW;
+    double bT[OCPX_NCMAX];       // OmegaB Tc / Pc: Bi = bT * P / T
 };
 
 struct DevSat {

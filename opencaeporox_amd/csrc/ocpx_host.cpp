@@ -86,6 +86,10 @@ struct ocpx_ctx {
     int pe_lanes = 1;            // batched-flash mapping: lanes per cell (1, 4, 8, 16)
     int pe_lanes_auto = 0;       // the autotuner's choice, 0 if it has not run
     int team_blocks[3] = {0, 0, 0}; // persistent grid of the team kernels, T = 4 / 8 / 16
+    int ncu = 256;               // compute units of the device
+    int stages = 1;              // batched flash: strict stability test in its own stages (pe_sta.hip)
+    bool sta_ready = false;      // the stage buffers are allocated
+    StaIO sta{};
     // Peaceman wells (ocpx_set_wells)
     int32_t nwell = 0, nperf = 0;
     ocpx_well* wells = nullptr;  // [nwell] device copy
@@ -251,6 +255,31 @@ struct TimerScope {
 
 inline int team_slot(int T) { return T == 4 ? 0 : (T == 8 ? 1 : 2); }
 
+// buffers of the stability stages, allocated at the first staged flash
+int sta_alloc(ocpx_ctx* c)
+{
+    if (c->sta_ready) return OCPX_OK;
+    const int64_t n = c->nb, cap = 4 * c->nb;
+    StaIO& s = c->sta;
+    unsigned* cnt = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &cnt, 4)) || (rc = dalloc(c, &s.glist, n)) || (rc = dalloc(c, &s.route, n)) ||
+        (rc = dalloc(c, &s.decided, n)) || (rc = dalloc(c, &s.defer, n)) || (rc = dalloc(c, &s.Ytd, n)) ||
+        (rc = dalloc(c, &s.evals, n)) || (rc = dalloc(c, &s.its, 4 * n)) || (rc = dalloc(c, &s.item, 4 * n)) ||
+        (rc = dalloc(c, &s.di, (size_t)c->NC * n)) || (rc = dalloc(c, &s.nr_cell, cap)) ||
+        (rc = dalloc(c, &s.nr_k, cap)) || (rc = dalloc(c, &s.nr_flag, cap)) || (rc = dalloc(c, &s.nr_its, cap)) ||
+        (rc = dalloc(c, &s.nr_evals, cap)) || (rc = dalloc(c, &s.nr_Yt, cap)) ||
+        (rc = dalloc(c, &s.nr_y, (size_t)c->NC * cap)) || (rc = dalloc(c, &s.rs_Yt, n)) ||
+        (rc = dalloc(c, &s.rs_cnt, 4 * n)))
+        return rc;
+    s.qhead = cnt;
+    s.gcount = cnt + 1;
+    s.ncount = cnt + 2;
+    s.nr_cap = cap;
+    c->sta_ready = true;
+    return OCPX_OK;
+}
+
 int launch_pe(ocpx_ctx* c, int mode)
 {
     PeIO io{};
@@ -271,8 +300,23 @@ int launch_pe(ocpx_ctx* c, int mode)
     io.mid = c->mid;
     if (c->mid && (mode != OCPX_FLASH_BATCHED || c->pe_lanes == 1))
         return fail(c, OCPX_E_UNSUPPORTED, "contexts of several models flash in batched mode with a team mapping");
+    // single-model batched flash: the strict stability test in stages, then the
+    // general kernel on the cells that remain (the split, other flash types)
+    const bool staged = mode == OCPX_FLASH_BATCHED && c->stages && !c->mid && c->nb > 0;
+    if (staged) {
+        int rc = sta_alloc(c);
+        if (rc) return rc;
+    }
     {
         TimerScope t(c, 0);
+        if (staged) {
+            HIPCHK(c, run_pe_sta(c->NC, c->df, io, c->sta, c->ncu, c->st));
+            io.qorder = c->sta.glist;
+            io.nq = c->sta.gcount;
+            io.sta_route = c->sta.route;
+            io.rs_Yt = c->sta.rs_Yt;
+            io.rs_cnt = c->sta.rs_cnt;
+        }
         if (mode == OCPX_FLASH_BATCHED && c->pe_lanes > 1) {
             const int T = c->pe_lanes;
             HIPCHK(c, run_pe_team(c->NC, T, c->df, io, true, c->team_blocks[team_slot(T)], c->st));
@@ -416,6 +460,7 @@ void build_dev_fluid(const ocpx_fluid* fl, DevFluid& h)
         h.kwA[i] = 5.373 * (1 + w);
         h.auxB[i] = 5.4402 * std::pow(fl->Tc[i], 1.0 / 6) / std::pow(fl->Pc[i], 2.0 / 3);
         h.sqrtMW[i] = std::sqrt(fl->MW[i]);
+        h.bT[i] = fl->OmegaB[i] * fl->Tc[i] / fl->Pc[i];
         for (int k = 0; k < fl->nc; k++) h.omk[i * OCPX_NCMAX + k] = 1 - fl->BIC[i * OCPX_MAX_NC + k];
     }
     for (int k = 0; k < 5; k++) { h.lbc[k] = fl->LBCcoef[k] * 10; h.pvtw[k] = fl->PVTW[k]; }
@@ -591,6 +636,7 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     int dev = 0, ncu = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    c->ncu = ncu > 0 ? ncu : 256;
     const int per_cu = pe_batched_blocks_per_cu(c->NC);
     c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
     for (int q = 0; q < 3; q++) {
@@ -812,6 +858,21 @@ int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
     c->pe_lanes_auto = best;
     c->qmode = nat > 0 && nat < best_ms ? 0 : 1;
     return ocpx_set_flash_mapping(c, best);
+}
+
+int ocpx_set_flash_stages(ocpx_ctx* c, int on)
+{
+    if (!c || (on != 0 && on != 1)) return OCPX_E_ARG;
+    if (on != c->stages) destroy_graph(c);
+    c->stages = on;
+    return OCPX_OK;
+}
+
+int ocpx_get_flash_stages(ocpx_ctx* c, int32_t* on)
+{
+    if (!c || !on) return OCPX_E_ARG;
+    *on = c->stages;
+    return OCPX_OK;
 }
 
 int ocpx_set_flash_queue_order(ocpx_ctx* c, int longest_first)

@@ -36,7 +36,36 @@ struct PeIO {
     unsigned* queue;            // work-queue counter (batched mode)
     const int32_t* qorder;      // [nb] cell handed out at each queue position (batched mode), or null
     const int32_t* mid;         // [nb] model of each cell (multi-model contexts), or null
+    const unsigned* nq;         // queue length on the device (batched mode), or null (nb)
+    // cells the stability stages found unstable (route 2) start at the split
+    // with the deciding trial's Yt and counters (pe_sta.hip), or null
+    const int32_t* sta_route;   // [nb]
+    const double* rs_Yt;        // [nb]
+    const int32_t* rs_cnt;      // [4][nb]: ITERS rows 0, 1, 5 and 6 (= row 7)
 };
+
+// stability stages of the batched flash (pe_sta.hip)
+struct StaIO {
+    unsigned* qhead;            // counters, three consecutive words: stage-1 queue head,
+    unsigned* gcount;           //   cells for the general kernel,
+    unsigned* ncount;           //   queued Newton trials
+    int32_t* glist;             // [nb] cells for the general kernel
+    int32_t* route;             // [nb] 0 general kernel, 1 stable/undecided by stage 1, 2 split
+    int32_t* decided;           // [nb] first trial stage 1 found unstable, 4 none
+    int32_t* defer;             // [nb] bit k: trial k queued for its Newton step
+    double* Ytd;                // [nb] Yt of the decided trial
+    int32_t* evals;             // [nb] stage-1 fugacity evaluations
+    int32_t* its;               // [4][nb] SSM iterations per trial
+    int32_t* item;              // [4][nb] Newton item of a queued trial
+    double* di;                 // [Nc][nb] phi * z of the feed
+    int64_t nr_cap;             // Newton items (4 nb)
+    int32_t *nr_cell, *nr_k, *nr_flag, *nr_its, *nr_evals;   // [nr_cap]
+    double *nr_Yt, *nr_y;       // [nr_cap], [Nc][nr_cap]
+    double* rs_Yt;              // [nb]
+    int32_t* rs_cnt;            // [4][nb]
+};
+// stages 1-3 of the batched flash for one model; ncu = compute units (persistent grids)
+hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int ncu, hipStream_t st);
 
 // queue order for the next batched flash (pe.hip): cells by descending count
 // of fugacity evaluations in the last one; hist has queue_order_buckets()

@@ -306,6 +306,18 @@ OCPX_HD void pe_begin(const DevFluid& f, const PeIO& io, int cell, PeLane<NC>& L
         split_start(f, L, io, false);
         return;
     }
+    if (io.sta_route && io.sta_route[cell] == 2) {   // unstable by the stability stages
+        L.cnt[0] = io.rs_cnt[cell];
+        L.cnt[1] = io.rs_cnt[n + cell];
+        L.cnt[5] = io.rs_cnt[2 * n + cell];
+        L.cnt[6] = L.cnt[7] = io.rs_cnt[3 * n + cell];
+        L.Yt = io.rs_Yt[cell];
+        L.NP = 2;                                  // strict_decide's unstable branch
+        L.stage = 0;
+        L.restarts = 0;
+        split_start(f, L, io, false);
+        return;
+    }
     UNROLL for (int i = 0; i < NC; i++) L.y[i] = L.z[i];
     L.st = ST_FEED;
 }
@@ -614,6 +626,7 @@ __global__ void __launch_bounds__(256) pe_batched(const DevFluid* __restrict__ f
     L.st = ST_DONE;
     bool have = false, drained = false;
     const int lane = __lane_id();
+    const unsigned nq = io.nq ? *io.nq : (unsigned)io.nb;
     while (true) {
         const bool need = (L.st == ST_DONE) && !drained;
         if (L.st == ST_DONE && have) { pe_end<NC>(io, L); have = false; }
@@ -625,7 +638,7 @@ __global__ void __launch_bounds__(256) pe_batched(const DevFluid* __restrict__ f
             base = __shfl(base, leader);
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-                if (idx < (unsigned)io.nb) {
+                if (idx < nq) {
                     pe_begin<NC>(f, io, io.qorder ? io.qorder[idx] : (int)idx, L);
                     have = true;
                 }

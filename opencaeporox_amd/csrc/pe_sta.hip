@@ -1,0 +1,437 @@
+// pe_sta.hip — the strict stability test of the batched flash as its own
+// stages, cells grouped by what they are doing.
+//
+// OCPPhaseEquilibrium::StableSSM (src/OCPPhaseEquilibrium.cpp:397-473) runs
+// four trial compositions in turn, each an SSM loop to SSMSTA's tolerance or
+// cap followed by StableNR (:475-521); the first trial whose NR converges
+// with Yt > 1 + eYt declares the feed unstable.  The SSM loops are most of a
+// flash's work, and they share one short body.  So:
+//
+//   stage 1  pe_sta_ssm   persistent lanes, one cell each: SkipPSA's flash
+//                         type, the feed's ln phi, then the trials' SSM loops
+//                         only.  A trial whose residual is already below
+//                         NRSTA's tolerance needs no Newton step and is
+//                         decided on the spot; any other is queued for stage 2
+//                         and the lane carries on with the next trial.  A lane
+//                         stops at the first trial it decides unstable.  Cells
+//                         that take another path (flash type 1 or 2, or the
+//                         relaxed test after a step with phases) are queued
+//                         for the general kernel at once.
+//   stage 2  pe_sta_nr    StableNR for the queued trials, one lane each.
+//   stage 3  pe_sta_join  per cell, the first trial in order that is unstable
+//                         (the reference's early exit), the counters of the
+//                         trials up to it; stable cells are written out,
+//                         unstable ones queued for the split in the general
+//                         kernel, which resumes them with that trial's Yt.
+//
+// Trials run past an undecided one are speculative: their results are not
+// used when the earlier trial turns out unstable, exactly as the reference
+// would not have run them.  Only multi-model contexts and the plain mode skip
+// these stages.
+#include "ocpx_dev.hpp"
+#include "ocpx_kernels.hpp"
+
+namespace ocpx {
+namespace sta {
+
+constexpr int NONE = 4;          // decided: no trial found unstable
+#ifndef STA_WAVES
+#define STA_WAVES 2               // stage-1 wavefronts per SIMD the register budget targets
+#endif
+#define DEV __device__ __forceinline__
+
+// SkipPSA::CalFtype02 (src/AcceleratePEC.cpp:103-124) and the previous phase
+// count, as the general kernels' begin step computes them
+template <int NC>
+OCPX_HD int flash_type(const DevFluid& f, const PeIO& io, int cell, const double* Ni, double Nt, double P,
+                       double Tk, uint32_t& lNP)
+{
+    const size_t n = io.nb;
+    uint32_t phaseNum = 0;
+    UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
+    lNP = phaseNum - 1u;
+    int ft = io.ftype_in ? io.ftype_in[cell] : -1;
+    if (ft >= 0) return ft;
+    ft = 0;
+    if (f.skip) {
+        bool skip = false;
+        if (io.skip_flag[cell]) {
+            const double me = io.skip_minEig[cell];
+            skip = fabs(1 - io.skip_P[cell] / P) < me / 10 && fabs(io.skip_T[cell] - Tk) < me * 10;
+            UNROLL for (int i = 0; i < NC; i++)
+                skip = skip && fabs(Ni[i] / Nt - io.skip_zi[(size_t)i * n + cell]) < me / 10;
+        }
+        const uint32_t npPE = phaseNum - 1u;
+        if (skip) ft = 1;
+        else if (npPE >= 2) {
+            const uint32_t t = (io.S[cell] >= 1E-4) + (io.S[n + cell] >= 1E-4);
+            ft = t == npPE ? (int)npPE : 0;
+        }
+    }
+    return ft;
+}
+
+// Wilson K of trial k, Kw[k] = Kw0^c_k with c = 1, -1, 1/3, -1/3
+// (CalKwilson, src/OCPPhaseEquilibrium.cpp:244-252)
+OCPX_HD double trial_coef(int k)
+{
+    return k == 0 ? 1.0 : (k == 1 ? -1.0 : (k == 2 ? (1.0 / 3) : -(1.0 / 3)));
+}
+
+OCPX_HD const DevFluid* opaque(const DevFluid* p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+// Append to a device list with one atomic per wavefront.
+__device__ __forceinline__ void append(bool want, unsigned* count, int32_t* list, int32_t value)
+{
+    const unsigned long long m = __ballot(want);
+    if (!m) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (want) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
+}
+
+enum : int { S_IDLE = 0, S_FEED, S_START, S_SSM };
+
+// ln phi at y (src/OCPEoS.cpp:68-124, 199-214 in log form) with the cell's
+// sqrt(Ai) in registers and Bi = bT P / T formed where used, so a lane holds
+// only what the SSM loop carries
+template <int NC>
+DEV void lean_lnphi(const DevFluid& f, const double* sA, double PT, const double* y, double* ev)
+{
+    double w[NC];
+    UNROLL for (int k = 0; k < NC; k++) w[k] = sA[k] * y[k];
+    double A = 0, B = 0;
+    UNROLL for (int i = 0; i < NC; i++) {
+        double t = 0;
+        UNROLL for (int k = 0; k < NC; k++) t += f.omk[i * OCPX_NCMAX + k] * w[k];
+        ev[i] = sA[i] * t;                                   // s_i for now
+        A += y[i] * ev[i];
+        B += (f.bT[i] * PT) * y[i];
+    }
+    const double Z = pr_z(A, B);
+    const double lz = log(Z - B);
+    const double lg = log((Z + D1 * B) / (Z + D2 * B));
+    const double cA = A / (D1 - D2) / B * lg;
+    const double iB = 1 / B, iA = 1 / A;
+    UNROLL for (int i = 0; i < NC; i++) {
+        const double Bi = f.bT[i] * PT;
+        ev[i] = Bi * iB * (Z - 1) - lz - cA * (2 * ev[i] * iA - Bi * iB);
+    }
+}
+
+// stage 1: one lane per cell, persistent over the work queue
+template <int NC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STA_WAVES)))
+pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
+{
+    const size_t n = io.nb;
+    const int lane = __lane_id();
+    int cell = -1, st = S_IDLE, k = 0, iter = 0, decided = NONE, defer = 0;
+    int its[4] = {0, 0, 0, 0};
+    double P = 0, PT = 0, logP = 0, Tk = 0, Yt = 0, Ytd = 0;
+    double y[NC], di[NC], lp[NC], sA[NC];
+    bool drained = false;
+    while (true) {
+        // cells for idle lanes; cells off the strict path go to the general list
+        while (true) {
+            const bool need = st == S_IDLE && !drained;
+            const unsigned long long m = __ballot(need);
+            if (!m) break;
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(s.qhead, (unsigned)__popcll(m));
+            base = __shfl(base, leader);
+            bool general = false;
+            const DevFluid& f0 = *opaque(fp);
+            if (need) {
+                const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                if (idx < (unsigned)n) {
+                    cell = io.qorder ? io.qorder[idx] : (int)idx;
+                    P = io.P[cell];
+                    Tk = io.T[cell] + CONV4;                          // SetVarSet :674
+                    double Ni[NC], Nt = 0;
+                    UNROLL for (int i = 0; i < NC; i++) { Ni[i] = io.Ni[(size_t)i * n + cell]; Nt += Ni[i]; }
+                    uint32_t lNP;
+                    const int ft = flash_type<NC>(f0, io, cell, Ni, Nt, P, Tk, lNP);
+                    if (ft == 0 && lNP == 0) {
+                        st = S_FEED;
+                        logP = log(P);
+                        PT = P / Tk;
+                        UNROLL for (int i = 0; i < NC; i++) {   // sqrt(Ai), src/OCPEoS.cpp:68-84
+                            const double Pr = P / f0.Pc[i], Tr = Tk / f0.Tc[i];
+                            const double q = 1 + f0.mw[i] * (1 - sqrt(Tr));
+                            sA[i] = sqrt(f0.OA[i] * Pr / (Tr * Tr) * (q * q));
+                            y[i] = Ni[i] / Nt;                             // InitNtZ :812-818
+                        }
+                        k = 0; iter = 0; decided = NONE; defer = 0;
+                        UNROLL for (int t = 0; t < 4; t++) its[t] = 0;
+                    } else {
+                        general = true;
+                        s.route[cell] = 0;
+                    }
+                } else {
+                    drained = true;
+                }
+            }
+            append(general, s.gcount, s.glist, cell);
+        }
+        if (!__any(st != S_IDLE)) break;
+        if (st == S_IDLE) continue;
+
+        const DevFluid& f = *opaque(fp);
+        // the composition evaluated in this step (the feed's is in y already)
+        if (st == S_START) {       // Y = x / Kw[k] (:420-425), x re-read from Ni
+            const double ck = trial_coef(k);
+            double Nt = 0;
+            UNROLL for (int i = 0; i < NC; i++) { y[i] = io.Ni[(size_t)i * n + cell]; Nt += y[i]; }
+            UNROLL for (int i = 0; i < NC; i++)
+                y[i] = (y[i] / Nt) / exp(ck * (f.logPc[i] - logP + f.kwA[i] * (1 - f.Tc[i] / Tk)));
+        } else if (st == S_SSM) {  // Y = di / phiSta (:439-444), lp = ln phiSta
+            UNROLL for (int i = 0; i < NC; i++) y[i] = di[i] * exp(-lp[i]);
+        }
+        if (st != S_FEED) {
+            double yt = 0;
+            UNROLL for (int i = 0; i < NC; i++) yt += y[i];
+            Yt = yt;
+            const double r = 1 / yt;
+            UNROLL for (int i = 0; i < NC; i++) y[i] *= r;
+        }
+        double ev[NC];
+        lean_lnphi<NC>(f, sA, PT, y, ev);
+
+        bool trial_end = false;
+        double Se = 0;
+        if (st == S_FEED) {        // di = phi[Id] x (:403-408), fug[Id] = di P
+            UNROLL for (int i = 0; i < NC; i++) {
+                di[i] = exp(ev[i]) * y[i];
+                s.di[(size_t)i * n + cell] = di[i];
+            }
+            st = S_START;
+        } else {
+            // Se = sum ln(fugSta / fugId * Yt)^2 (:429-432, :448-451); after an SSM
+            // update Y Yt = di / phiSta of the previous iterate, so the log is the
+            // change of ln phiSta
+            const bool first = st == S_START;
+            if (first) {
+                UNROLL for (int i = 0; i < NC; i++) {
+                    const double r = log((exp(ev[i]) * y[i] * P) / (di[i] * P) * Yt);
+                    Se += r * r;
+                }
+            } else {
+                UNROLL for (int i = 0; i < NC; i++) {
+                    const double r = ev[i] - lp[i];
+                    Se += r * r;
+                }
+            }
+            UNROLL for (int i = 0; i < NC; i++) lp[i] = ev[i];
+            if (first) {
+                iter = 0;
+                if (Se > f.SSMsta_tol2) st = S_SSM;
+                else trial_end = true;
+            } else {
+                its[k]++;
+                if (!(Se > f.SSMsta_tol2 && ++iter <= f.SSMsta_maxIt)) trial_end = true;
+            }
+        }
+        if (trial_end) {
+            // StableNR (:478-488): its residual is the last SSM residual
+            bool done = false;
+            if (sqrt(Se) > f.NRsta_tol) {
+                defer |= 1 << k;
+                const unsigned q = atomicAdd(s.ncount, 1u);
+                s.nr_cell[q] = cell;
+                s.nr_k[q] = k;
+                s.nr_Yt[q] = Yt;
+                UNROLL for (int i = 0; i < NC; i++) s.nr_y[(size_t)i * s.nr_cap + q] = y[i];
+                s.item[(size_t)k * n + cell] = (int32_t)q;
+            } else if (Yt > 1 + f.eYt) {                  // flag && Yt > 1 + eYt (:463)
+                decided = k;
+                Ytd = Yt;
+                done = true;
+            }
+            if (!done && ++k == 4) done = true;
+            if (done) {
+                s.route[cell] = 1;
+                s.decided[cell] = decided;
+                s.defer[cell] = defer;
+                s.Ytd[cell] = Ytd;
+                UNROLL for (int t = 0; t < 4; t++) s.its[(size_t)t * n + cell] = its[t];
+                st = S_IDLE;
+            } else {
+                st = S_START;
+            }
+        }
+    }
+}
+
+// stage 2: StableNR (:475-521) of a queued trial, one lane each, from its
+// SSM's last Y and Yt.  Jacobian JmatSTA (:523-541) solved as the symmetric
+// matrix of its lower triangle (dsysv_ 'U' on row-major storage, :493).
+template <int NC>
+__global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
+{
+    const unsigned total = *s.ncount;
+    const size_t n = io.nb;
+    for (unsigned q = blockIdx.x * 64 + threadIdx.x; q < total; q += gridDim.x * 64) {
+        const DevFluid& f = *opaque(fp);
+        const int cell = s.nr_cell[q];
+        const double P = io.P[cell], Tk = io.T[cell] + CONV4;
+        EosCell<NC> ec;
+        eos_cell<NC>(f, P, Tk, ec);
+        double y[NC], di[NC], res[NC];
+        double Yt = s.nr_Yt[q];
+        UNROLL for (int i = 0; i < NC; i++) {
+            y[i] = s.nr_y[(size_t)i * s.nr_cap + q];
+            di[i] = s.di[(size_t)i * n + cell];
+        }
+        EosMix<NC> mx;
+        double ev[NC];
+        eos_mix<NC>(f, ec, y, mx);
+        eos_lnphi<NC>(ec, mx, ev);
+        int evals = 1;
+        double e2 = 0;
+        UNROLL for (int i = 0; i < NC; i++) {       // resSTA = ln(fug[Id] / (fugSta Yt))
+            res[i] = log((di[i] * P) / ((exp(ev[i]) * y[i] * P) * Yt));
+            e2 += res[i] * res[i];
+        }
+        int iter = 0, flag = 1;
+        while (sqrt(e2) > f.NRsta_tol) {
+            double Zx[NC];
+            const double den = z_den(mx), bc = z_bcoef(mx);
+            UNROLL for (int j = 0; j < NC; j++) Zx[j] = ((mx.B - mx.Z) * (2 * mx.s[j]) + bc * ec.Bi[j]) / den;
+            double S[NC][NC], rhs[1][NC];
+            UNROLL for (int i = 0; i < NC; i++) {
+                double row[NC];
+                lnfugx_row<NC>(f, ec, mx, Zx, y, P, i, row);
+                double t = 0;
+                UNROLL for (int j = 0; j < NC; j++) t += y[j] * row[j];
+                UNROLL for (int j = 0; j <= i; j++) {
+                    const double v = (row[j] - t + 1) / Yt;
+                    S[i][j] = v;
+                    S[j][i] = v;
+                }
+                rhs[0][i] = res[i];
+            }
+            gauss_solve<NC, 1>(S, rhs);
+            const double lYt = Yt;
+            double yt = 0;
+            UNROLL for (int i = 0; i < NC; i++) { y[i] = y[i] * lYt + rhs[0][i]; yt += y[i]; }
+            Yt = yt;
+            const double r = 1 / yt;
+            UNROLL for (int i = 0; i < NC; i++) y[i] *= r;
+            eos_mix<NC>(f, ec, y, mx);
+            eos_lnphi<NC>(ec, mx, ev);
+            evals++;
+            e2 = 0;
+            UNROLL for (int i = 0; i < NC; i++) {
+                res[i] = log((di[i] * P) / ((exp(ev[i]) * y[i] * P) * Yt));
+                e2 += res[i] * res[i];
+            }
+            if (++iter > f.NRsta_maxIt) { flag = 0; break; }
+        }
+        s.nr_flag[q] = flag;
+        s.nr_Yt[q] = Yt;
+        s.nr_its[q] = iter;
+        s.nr_evals[q] = evals - 1;              // the re-evaluation at the SSM's Y is not the reference's
+    }
+}
+
+// stage 3: the reference's order of trials, the counters up to the deciding
+// trial, and the split queue
+template <int NC>
+__global__ void __launch_bounds__(256) pe_sta_join(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
+{
+    const size_t n = io.nb;
+    const int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool split = false;
+    if (cell < (int64_t)n && s.route[cell] == 1) {
+        const DevFluid& f = *fp;
+        const int dec = s.decided[cell], def = s.defer[cell];
+        int kstar = NONE;
+        double Yt = 0;
+        int c0 = 0, c1 = 0, c5 = 1, c6 = 0;      // c5: the feed's evaluation
+        for (int k = 0; k < 4; k++) {
+            if (k > dec) break;
+            const int it = s.its[(size_t)k * n + cell];
+            c0 += it;
+            c5 += 1 + it;
+            if (k == dec) { kstar = k; Yt = s.Ytd[cell]; break; }
+            if (def & (1 << k)) {
+                const unsigned q = (unsigned)s.item[(size_t)k * n + cell];
+                const int nit = s.nr_its[q];
+                c1 += nit;
+                c5 += s.nr_evals[q];
+                c6 += nit;
+                if (s.nr_flag[q] && s.nr_Yt[q] > 1 + f.eYt) { kstar = k; Yt = s.nr_Yt[q]; break; }
+            }
+        }
+        if (kstar == NONE) {                       // stable: NP = 1, x = z
+            double Ni[NC], Nt = 0;
+            UNROLL for (int i = 0; i < NC; i++) { Ni[i] = io.Ni[(size_t)i * n + cell]; Nt += Ni[i]; }
+            io.pe_NP[cell] = 1;
+            io.pe_nu[cell] = 1.0;
+            UNROLL for (int i = 0; i < NC; i++) {
+                io.pe_x[(size_t)i * n + cell] = Ni[i] / Nt;
+                io.pe_x[(size_t)(NC + i) * n + cell] = 0.0;
+            }
+            io.ftype_out[cell] = 0;
+            if (io.iters) {
+                const int cnt[8] = {c0, c1, 0, 0, 0, c5, c6, c6};
+                UNROLL for (int q = 0; q < 8; q++) io.iters[(size_t)q * n + cell] = cnt[q];
+            }
+        } else {                                   // unstable at trial kstar: split
+            s.rs_Yt[cell] = Yt;
+            s.rs_cnt[(size_t)0 * n + cell] = c0;
+            s.rs_cnt[(size_t)1 * n + cell] = c1;
+            s.rs_cnt[(size_t)2 * n + cell] = c5;
+            s.rs_cnt[(size_t)3 * n + cell] = c6;
+            s.route[cell] = 2;
+            split = true;
+        }
+    }
+    append(split, s.gcount, s.glist, (int32_t)cell);
+}
+
+__global__ void __launch_bounds__(64) sta_reset(unsigned* c)
+{
+    if (threadIdx.x < 3) c[threadIdx.x] = 0;
+}
+
+template <int NC>
+hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, int ncu, hipStream_t st)
+{
+    if (io.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(sta_reset, dim3(1), dim3(64), 0, st, s.qhead);
+    int b1 = 0, b2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, pe_sta_ssm<NC>, 64, 0) != hipSuccess || b1 <= 0) b1 = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, pe_sta_nr<NC>, 64, 0) != hipSuccess || b2 <= 0) b2 = 2;
+    const long need = (io.nb + 63) / 64;
+    const long g1 = (long)ncu * b1 < need ? (long)ncu * b1 : need;
+    const long g2 = (long)ncu * b2 < need ? (long)ncu * b2 : need;
+    hipLaunchKernelGGL(pe_sta_ssm<NC>, dim3((unsigned)g1), dim3(64), 0, st, f, io, s);
+    hipLaunchKernelGGL(pe_sta_nr<NC>, dim3((unsigned)g2), dim3(64), 0, st, f, io, s);
+    hipLaunchKernelGGL(pe_sta_join<NC>, dim3((unsigned)((io.nb + 255) / 256)), dim3(256), 0, st, f, io, s);
+    return hipGetLastError();
+}
+
+}  // namespace sta
+
+hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int ncu, hipStream_t st)
+{
+    switch (NC) {
+#define OCPX_CASE(N) case N: return sta::launch<N>(f, io, s, ncu, st);
+        OCPX_CASE(2) OCPX_CASE(3) OCPX_CASE(4) OCPX_CASE(5) OCPX_CASE(6) OCPX_CASE(7)
+        OCPX_CASE(8) OCPX_CASE(9) OCPX_CASE(10) OCPX_CASE(11) OCPX_CASE(12)
+#undef OCPX_CASE
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace ocpx

@@ -563,6 +563,18 @@ DEV void begin(const DevFluid& f, LaneFluid<NC, T>& lf, const PeIO& io, int cell
         split_start(f, lf, C, io, false, ln);
         return;
     }
+    if (io.sta_route && io.sta_route[cell] == 2) {   // unstable by the stability stages
+        C.cnt[0] = io.rs_cnt[cell];
+        C.cnt[1] = io.rs_cnt[n + cell];
+        C.cnt[5] = io.rs_cnt[2 * n + cell];
+        C.cnt[6] = C.cnt[7] = io.rs_cnt[3 * n + cell];
+        C.Yt = io.rs_Yt[cell];
+        C.NP = 2;                                  // strict_decide's unstable branch
+        C.stage = 0;
+        C.restarts = 0;
+        split_start(f, lf, C, io, false, ln);
+        return;
+    }
     SLOTS C.y[c] = C.z[c];
     C.st = ST_FEED;
 }
@@ -941,6 +953,7 @@ __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict
     C.st = ST_DONE;
     C.model = 0;
     bool have = false, drained = false;
+    const unsigned nq = io.nq ? *io.nq : (unsigned)io.nb;
     while (true) {
         if (C.st == ST_DONE && have) { end<NC, T>(io, C, ln); have = false; }
         const bool need = !have && !drained;
@@ -952,7 +965,7 @@ __global__ void __launch_bounds__(64) pe_team_batched(const DevFluid* __restrict
             base = __shfl(base, first);
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lead) - 1ull));
-                if (idx < (unsigned)io.nb) {
+                if (idx < nq) {
                     const int cell = io.qorder ? io.qorder[idx] : (int)idx;
                     C.model = MULTI ? io.mid[cell] : 0;
                     begin<NC, T>(fp[C.model], lf, io, cell, C, ln);

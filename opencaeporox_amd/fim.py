@@ -163,6 +163,15 @@ class IsoTFIM:
         self._chk(self.lib.ocpx_get_flash_queue_order(self.h, C.byref(v)), "queue order")
         return bool(v.value)
 
+    def set_flash_stages(self, on):
+        """Strict stability test of the batched flash in its own stages (default on)."""
+        self._chk(self.lib.ocpx_set_flash_stages(self.h, 1 if on else 0), "flash stages")
+
+    def flash_stages(self):
+        v = C.c_int32()
+        self._chk(self.lib.ocpx_get_flash_stages(self.h, C.byref(v)), "flash stages")
+        return bool(v.value)
+
     def CalFlash(self, mode=PLAIN):
         self._chk(self.lib.ocpx_flash_der(self.h, mode), "flash")
 

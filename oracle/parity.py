@@ -69,6 +69,27 @@ def rounding_spread(st, others, field, n, rowwise=False, floor=FLOOR):
     return spread
 
 
+# Trajectories through degenerate iterates: a Rachford-Rice fraction outside
+# [-10, 11] (bit 32) makes the phase compositions z / (1 + nu (K - 1))
+# negative or huge, and their fugacities overflow or turn non-finite (bit
+# 16).  How such a trajectory continues depends on the order in which each
+# operation rounds and overflows: the kernels' own logic stepped on the host
+# with fused multiply-adds flips labels in cells that the oracle's
+# perturbations leave alone.  Labels there are compared but not required to
+# agree; the tests bound how many differ.
+DEGENERATE = 16 | 32
+
+
+def degenerate(ill):
+    return (np.asarray(ill) & DEGENERATE) != 0
+
+
+def cell_spread(spread):
+    """The largest entry of each cell's spread: how far rounding moves the
+    cell's field as a whole, which its solves' conditioning sets."""
+    return np.broadcast_to(np.max(spread, axis=1, keepdims=True), spread.shape)
+
+
 def label_stable(st, others):
     """Cells whose phase labels the oracle reproduces under every rounding-
     level perturbation in others: the labels the reference algorithm itself
@@ -83,10 +104,13 @@ def label_stable(st, others):
 
 def perturbed_oracle_states(oracle, oracle_fma, fluid, st_pre):
     """Flash outputs of the oracle under rounding-level perturbations of the
-    pre-flash state st_pre (left untouched)."""
-    a = st_pre.copy()
-    oracle_fma.cal_flash(fluid, a)
-    b = st_pre.copy()
-    b.a["P"] *= 1 + 2.0 ** -50
-    oracle.cal_flash(fluid, b)
-    return [a, b]
+    pre-flash state st_pre (left untouched): its fused-multiply-add build, and
+    the pressures scaled by 1 + 2^-50 and 1 - 2^-50 (plain and fused builds)."""
+    out = []
+    for orc, f in ((oracle_fma, 1.0), (oracle, 1 + 2.0 ** -50), (oracle, 1 - 2.0 ** -50),
+                   (oracle_fma, 1 - 2.0 ** -50)):
+        a = st_pre.copy()
+        a.a["P"] *= f
+        orc.cal_flash(fluid, a)
+        out.append(a)
+    return out

@@ -52,5 +52,5 @@ def close(a, b, rtol=1e-10, scale=None):
     return bool(np.all(ok)), float(np.max(err / np.maximum(np.abs(b) + scale, 1e-300))) if b.size else 0.0
 
 
-from oracle.parity import (FLOOR, PARITY_RTOL, SPLIT_ILL, SPREAD_MULT, label_stable,  # noqa: E402,F401
-                           perturbed_oracle_states, rel_err, rounding_sensitive, rounding_spread)
+from oracle.parity import (FLOOR, PARITY_RTOL, SPLIT_ILL, SPREAD_MULT, cell_spread, degenerate,  # noqa: E402,F401
+                           label_stable, perturbed_oracle_states, rel_err, rounding_sensitive, rounding_spread)
