@@ -16,6 +16,7 @@
 #include "../../include/ocpx.h"
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
+#include "layout.hpp"
 
 namespace ocpx {
 int64_t res_asm_blocks(int nc, int64_t nb);
@@ -87,6 +88,7 @@ struct ocpx_ctx {
     int pe_lanes_auto = 0;       // the autotuner's choice, 0 if it has not run
     int team_blocks[3] = {0, 0, 0}; // persistent grid of the team kernels, T = 4 / 8 / 16
     int ncu = 256;               // compute units of the device
+    int sta_grid[2] = {0, 0};    // persistent grids of the stability stages (SSM, Newton)
     int stages = 1;              // batched flash: strict stability test in its own stages (pe_sta.hip)
     bool sta_ready = false;      // the stage buffers are allocated
     StaIO sta{};
@@ -310,7 +312,7 @@ int launch_pe(ocpx_ctx* c, int mode)
     {
         TimerScope t(c, 0);
         if (staged) {
-            HIPCHK(c, run_pe_sta(c->NC, c->df, io, c->sta, c->ncu, c->st));
+            HIPCHK(c, run_pe_sta(c->NC, c->df, io, c->sta, c->sta_grid[0], c->sta_grid[1], c->st));
             io.qorder = c->sta.glist;
             io.nq = c->sta.gcount;
             io.sta_route = c->sta.route;
@@ -340,45 +342,6 @@ int default_lanes(int NC, bool multi)
 {
     if (NC <= 4) return multi ? 4 : 1;
     return NC <= 8 ? 8 : 16;
-}
-
-// Row order for the assembly.  When the connections are those of a natural
-// Cartesian numbering (every e - b is 1, nx or nx*ny, x and y connections do
-// not wrap), rows go in pencils: 16 x 16 tiles of the xy plane, each swept
-// through z, so the z-neighbours a row reads were touched a plane earlier
-// and are still in the XCD's L2.  Otherwise the natural order is kept.
-std::vector<int32_t> pencil_order(int64_t nb, int64_t nconn, const int32_t* cb, const int32_t* ce)
-{
-    int64_t d[3] = {0, 0, 0};
-    int nd = 0;
-    for (int64_t k = 0; k < nconn; k++) {
-        const int64_t v = (int64_t)ce[k] - cb[k];
-        bool seen = false;
-        for (int q = 0; q < nd; q++) seen = seen || d[q] == v;
-        if (seen) continue;
-        if (nd == 3 || v <= 0) return {};
-        d[nd++] = v;
-    }
-    if (nd != 3) return {};
-    std::sort(d, d + 3);
-    const int64_t nx = d[1], nxy = d[2];
-    if (d[0] != 1 || nxy % nx != 0 || nb % nxy != 0) return {};
-    const int64_t ny = nxy / nx, nz = nb / nxy;
-    if (nz < 2 || ny < 2 || nx < 2) return {};
-    for (int64_t k = 0; k < nconn; k++) {
-        const int64_t b = cb[k], e = ce[k];
-        if (e - b == 1 && b % nx == nx - 1) return {};
-        if (e - b == nx && (b / nx) % ny == ny - 1) return {};
-    }
-    const int64_t TX = 16, TY = 16;
-    std::vector<int32_t> ord;
-    ord.reserve(nb);
-    for (int64_t y0 = 0; y0 < ny; y0 += TY)
-        for (int64_t x0 = 0; x0 < nx; x0 += TX)
-            for (int64_t z = 0; z < nz; z++)
-                for (int64_t y = y0; y < std::min(ny, y0 + TY); y++)
-                    for (int64_t x = x0; x < std::min(nx, x0 + TX); x++) ord.push_back((int32_t)(z * nxy + y * nx + x));
-    return ord;
 }
 
 int launch_tail(ocpx_ctx* c)
@@ -435,6 +398,9 @@ int destroy_graph(ocpx_ctx* c)
     return OCPX_OK;
 }
 
+}  // namespace
+
+namespace ocpx {
 // Device fluid constants from the ABI description (EoS_PR ctor
 // src/OCPEoS.cpp:19-65, ViscosityMethod03 ctor src/OCPFuncPVT.cpp:415-459,
 // OCPPhaseEquilibrium::Setup src/OCPPhaseEquilibrium.cpp:45-64)
@@ -466,6 +432,9 @@ void build_dev_fluid(const ocpx_fluid* fl, DevFluid& h)
     for (int k = 0; k < 5; k++) { h.lbc[k] = fl->LBCcoef[k] * 10; h.pvtw[k] = fl->PVTW[k]; }
     h.rhoW = fl->rhoW_std;
 }
+}  // namespace ocpx
+
+namespace {
 
 }  // namespace
 
@@ -487,6 +456,15 @@ int create_fail(int code, const char* msg)
     g_create_err = msg;
     return code;
 }
+}  // namespace
+
+namespace ocpx {
+// the message ocpx_last_error(NULL) returns, for calls without a context
+void set_call_error(const std::string& msg) { g_create_err = msg; }
+void clear_call_error() { g_create_err.clear(); }
+}  // namespace ocpx
+
+namespace {
 
 int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const ocpx_rock* rock,
                     const ocpx_grid* g, const int32_t* mid, void* stream, ocpx_ctx** out)
@@ -585,21 +563,8 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     // index order, row e gets (e, b) and row b gets (b, e)
     c->nnzb = c->nb + 2 * c->nconn;
     std::vector<int64_t> rp(c->nb + 1, 0);
-    std::vector<int64_t> cnt(c->nb, 1);
-    for (int64_t k = 0; k < c->nconn; k++) { cnt[g->conn_b[k]]++; cnt[g->conn_e[k]]++; }
-    for (int64_t n = 0; n < c->nb; n++) rp[n + 1] = rp[n] + cnt[n];
     std::vector<int32_t> col(c->nnzb), ts(c->nnzb), nbc(c->nnzb);
-    std::vector<int64_t> fill(c->nb);
-    for (int64_t n = 0; n < c->nb; n++) {
-        col[rp[n]] = (int32_t)n; ts[rp[n]] = (int32_t)rp[n]; nbc[rp[n]] = -1; fill[n] = rp[n] + 1;
-    }
-    for (int64_t k = 0; k < c->nconn; k++) {
-        const int32_t b = g->conn_b[k], e = g->conn_e[k];
-        const int64_t se = fill[e]++, sb = fill[b]++;
-        col[se] = b; nbc[se] = (int32_t)k;
-        col[sb] = e; nbc[sb] = (int32_t)k;
-        ts[se] = (int32_t)sb; ts[sb] = (int32_t)se;
-    }
+    bsr_layout(c->nb, c->nconn, g->conn_b, g->conn_e, rp.data(), col.data(), ts.data(), nbc.data());
     const int B = c->nc + 1;
     ALLOC(c->rowptr, c->nb + 1); ALLOC(c->colind, c->nnzb); ALLOC(c->tslot, c->nnzb); ALLOC(c->nbr_conn, c->nnzb);
     ALLOC(c->val, (size_t)c->nnzb * B * B); ALLOC(c->rhs, (size_t)c->nb * B); ALLOC(c->resAbs, (size_t)c->nb * B);
@@ -637,6 +602,7 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     c->ncu = ncu > 0 ? ncu : 256;
+    for (int q = 0; q < 2; q++) c->sta_grid[q] = c->ncu * pe_sta_blocks_per_cu(c->NC, q);
     const int per_cu = pe_batched_blocks_per_cu(c->NC);
     c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
     for (int q = 0; q < 3; q++) {
@@ -1040,19 +1006,11 @@ int ocpx_set_wells(ocpx_ctx* c, int32_t nwell, const ocpx_well* wells, const int
         return rc;
     {
         // perforation -> well, and the perforated cells with their perforations in order
-        std::vector<int32_t> pw(np_);
-        for (int32_t w = 0; w < nwell; w++)
-            for (int32_t p = wells[w].perf0; p < wells[w].perf0 + wells[w].nperf; p++) pw[p] = w;
-        std::vector<std::pair<int64_t, int32_t>> byc(np_);
-        for (int64_t p = 0; p < np_; p++) byc[p] = {perf_cell[p], (int32_t)p};
-        std::sort(byc.begin(), byc.end());
+        std::vector<int32_t> pw, up, upf;
         std::vector<int64_t> uc;
-        std::vector<int32_t> up, upf(np_);
-        for (int64_t k = 0; k < np_; k++) {
-            if (k == 0 || byc[k].first != byc[k - 1].first) { uc.push_back(byc[k].first); up.push_back((int32_t)k); }
-            upf[k] = byc[k].second;
-        }
-        up.push_back((int32_t)np_);
+        std::vector<int32_t> w0(nwell), wn(nwell);
+        for (int32_t w = 0; w < nwell; w++) { w0[w] = wells[w].perf0; wn[w] = wells[w].nperf; }
+        well_layout(nwell, w0.data(), wn.data(), np_, perf_cell, pw, uc, up, upf);
         if ((rc = dalloc(c, &c->ucell, uc.size())) || (rc = dalloc(c, &c->uptr, up.size()))) return rc;
         HIPCHK(c, hipMemcpy(c->pwell, pw.data(), 4 * pw.size(), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->uperf, upf.data(), 4 * upf.size(), hipMemcpyHostToDevice));
@@ -1446,6 +1404,10 @@ int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
     if (c->own_stream == false && c->st == nullptr) return fail(c, OCPX_E_STATE, "graph capture needs a stream");
     if (!c->gexec || c->gdt != dt || c->gmode != mode) {
         destroy_graph(c);
+        if (mode == OCPX_FLASH_BATCHED && c->stages && !c->mid) {   // no allocation inside the capture
+            int rc = sta_alloc(c);
+            if (rc) return rc;
+        }
         const bool tm = c->timing;
         c->timing = false;
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));

@@ -19,7 +19,7 @@ struct PeIO {
     const double* P;            // [nb]
     const double* T;            // [nb] degC
     const double* Ni;           // [nc][nb]
-    const uint32_t* phaseExist; // [3][nb]  previous step
+    const uint32_t* phaseExist; // [3][nb]  previous step, or null (one phase: the strict test)
     const double* S;            // [3][nb]  previous step
     const double* xij;          // [3*nc][nb] previous step
     const int32_t* ftype_in;    // [nb] or null; < 0 -> SkipPSA
@@ -44,6 +44,11 @@ struct PeIO {
     const int32_t* rs_cnt;      // [4][nb]: ITERS rows 0, 1, 5 and 6 (= row 7)
 };
 
+// device fluid constants of one PVT region (ocpx_host.cpp)
+void build_dev_fluid(const ocpx_fluid* fl, DevFluid& h);
+// resident 64-lane blocks of pe_batched per compute unit (pe.hip)
+int pe_batched_blocks_per_cu(int NC);
+
 // stability stages of the batched flash (pe_sta.hip)
 struct StaIO {
     unsigned* qhead;            // counters, three consecutive words: stage-1 queue head,
@@ -64,8 +69,12 @@ struct StaIO {
     double* rs_Yt;              // [nb]
     int32_t* rs_cnt;            // [4][nb]
 };
-// stages 1-3 of the batched flash for one model; ncu = compute units (persistent grids)
-hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int ncu, hipStream_t st);
+// stages 1-3 of the batched flash for one model on persistent grids of grid1
+// (SSM loops) and grid2 (Newton steps) 64-lane blocks
+hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int grid1, int grid2,
+                      hipStream_t st);
+// resident blocks per compute unit of the SSM (which = 0) and Newton (1) stages
+int pe_sta_blocks_per_cu(int NC, int which);
 
 // queue order for the next batched flash (pe.hip): cells by descending count
 // of fugacity evaluations in the last one; hist has queue_order_buckets()

@@ -264,8 +264,11 @@ OCPX_HD void pe_begin(const DevFluid& f, const PeIO& io, int cell, PeLane<NC>& L
     double Nt = 0;
     UNROLL for (int i = 0; i < NC; i++) { Ni[i] = io.Ni[(size_t)i * n + cell]; Nt += Ni[i]; }
     UNROLL for (int i = 0; i < NC; i++) L.z[i] = Ni[i] / Nt;         // InitNtZ :812-818
-    uint32_t phaseNum = 0;
-    UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
+    uint32_t phaseNum = 1;                         // no previous step: one phase
+    if (io.phaseExist) {
+        phaseNum = 0;
+        UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
+    }
     L.lNP = phaseNum - 1u;
     // flash type: explicit override or SkipPSA::CalFtype02 (src/AcceleratePEC.cpp:103-124)
     int ft = io.ftype_in ? io.ftype_in[cell] : -1;

@@ -47,8 +47,11 @@ OCPX_HD int flash_type(const DevFluid& f, const PeIO& io, int cell, const double
                        double Tk, uint32_t& lNP)
 {
     const size_t n = io.nb;
-    uint32_t phaseNum = 0;
-    UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
+    uint32_t phaseNum = 1;                         // no previous step: one phase
+    if (io.phaseExist) {
+        phaseNum = 0;
+        UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
+    }
     lNP = phaseNum - 1u;
     int ft = io.ftype_in ? io.ftype_in[cell] : -1;
     if (ft >= 0) return ft;
@@ -405,16 +408,22 @@ __global__ void __launch_bounds__(64) sta_reset(unsigned* c)
 }
 
 template <int NC>
-hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, int ncu, hipStream_t st)
+int blocks_per_cu(int which)
+{
+    int b = 0;
+    const hipError_t e = which == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_ssm<NC>, 64, 0)
+                                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_nr<NC>, 64, 0);
+    return e == hipSuccess && b > 0 ? b : 2;
+}
+
+template <int NC>
+hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, int grid1, int grid2, hipStream_t st)
 {
     if (io.nb == 0) return hipSuccess;
     hipLaunchKernelGGL(sta_reset, dim3(1), dim3(64), 0, st, s.qhead);
-    int b1 = 0, b2 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, pe_sta_ssm<NC>, 64, 0) != hipSuccess || b1 <= 0) b1 = 2;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b2, pe_sta_nr<NC>, 64, 0) != hipSuccess || b2 <= 0) b2 = 2;
     const long need = (io.nb + 63) / 64;
-    const long g1 = (long)ncu * b1 < need ? (long)ncu * b1 : need;
-    const long g2 = (long)ncu * b2 < need ? (long)ncu * b2 : need;
+    const long g1 = grid1 < need ? grid1 : need;
+    const long g2 = grid2 < need ? grid2 : need;
     hipLaunchKernelGGL(pe_sta_ssm<NC>, dim3((unsigned)g1), dim3(64), 0, st, f, io, s);
     hipLaunchKernelGGL(pe_sta_nr<NC>, dim3((unsigned)g2), dim3(64), 0, st, f, io, s);
     hipLaunchKernelGGL(pe_sta_join<NC>, dim3((unsigned)((io.nb + 255) / 256)), dim3(256), 0, st, f, io, s);
@@ -423,14 +432,26 @@ hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, int ncu, hi
 
 }  // namespace sta
 
-hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int ncu, hipStream_t st)
+hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int grid1, int grid2,
+                      hipStream_t st)
 {
     switch (NC) {
-#define OCPX_CASE(N) case N: return sta::launch<N>(f, io, s, ncu, st);
+#define OCPX_CASE(N) case N: return sta::launch<N>(f, io, s, grid1, grid2, st);
         OCPX_CASE(2) OCPX_CASE(3) OCPX_CASE(4) OCPX_CASE(5) OCPX_CASE(6) OCPX_CASE(7)
         OCPX_CASE(8) OCPX_CASE(9) OCPX_CASE(10) OCPX_CASE(11) OCPX_CASE(12)
 #undef OCPX_CASE
     default: return hipErrorInvalidValue;
+    }
+}
+
+int pe_sta_blocks_per_cu(int NC, int which)
+{
+    switch (NC) {
+#define OCPX_CASE(N) case N: return sta::blocks_per_cu<N>(which);
+        OCPX_CASE(2) OCPX_CASE(3) OCPX_CASE(4) OCPX_CASE(5) OCPX_CASE(6) OCPX_CASE(7)
+        OCPX_CASE(8) OCPX_CASE(9) OCPX_CASE(10) OCPX_CASE(11) OCPX_CASE(12)
+#undef OCPX_CASE
+    default: return 2;
     }
 }
 

@@ -512,8 +512,11 @@ DEV void begin(const DevFluid& f, LaneFluid<NC, T>& lf, const PeIO& io, int cell
     SLOTS Ni[c] = ln.act[c] ? io.Ni[(size_t)(c * T + ln.t) * n + cell] : 0.0;
     const double Nt = TSUM(Ni[c]);
     SLOTS C.z[c] = ln.act[c] ? Ni[c] / Nt : 0.0;
-    uint32_t phaseNum = 0;
-    UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
+    uint32_t phaseNum = 1;                         // no previous step: one phase
+    if (io.phaseExist) {
+        phaseNum = 0;
+        UNROLL for (int j = 0; j < 3; j++) phaseNum += io.phaseExist[(size_t)j * n + cell] != 0;
+    }
     C.lNP = phaseNum - 1u;
     int ft = io.ftype_in ? io.ftype_in[cell] : -1;
     if (ft < 0) {
