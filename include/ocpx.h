@@ -242,6 +242,24 @@ int  ocpx_flash_der(ocpx_ctx* ctx, int mode);
 int  ocpx_phase_equilibrium(ocpx_ctx* ctx, int mode);
 /* NP [nb] int32, nu [nb*2], x [nb*2*Nc] (PE order, before relabelling). */
 int  ocpx_get_pe_result(ocpx_ctx* ctx, int32_t* NP, double* nu, double* x);
+/* Phase equilibrium of n cells that allocates nothing.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It runs the kernel of ocpx_phase_equilibrium's batched mode
+ * with one cell per lane (no stability stages) on every cell from one phase
+ * (OCPPhaseEquilibrium::PhaseEquilibrium(P, T, z) with ftype 0,
+ * src/OCPPhaseEquilibrium.cpp:71-112), on device arrays the caller owns:
+ * P [n] bar, T [n] degC, z [Nc][n] (component-major, normalised by the
+ * kernel); outputs NP [n], nu [n] (nu of PE phase 0; phase 1 has 1 - nu),
+ * x [2*Nc][n] (phase 0's Nc fractions, then phase 1's) and ftype [n],
+ * device-side and component-major.  Its scratch (the work-queue counter and the fluid
+ * constants) is a fixed 8 KiB device array per GPU, held until the process
+ * ends; calls that overlap on one GPU race.  Limits: Nc 2..12, n up to
+ * 2^31 - 1 cells; anything else returns OCPX_E_ARG, and ocpx_last_error(NULL)
+ * says why.  Asynchronous on hip_stream (NULL: the default stream). */
+int  ocpx_phase_equilibrium_noalloc(const ocpx_fluid* fluid, int64_t n, const double* P, const double* T,
+                                    const double* z, int32_t* NP, double* nu, double* x, int32_t* ftype,
+                                    void* hip_stream);
 
 /* Cell-to-lane mapping of the batched flash's phase-equilibrium kernel.  No
  * counterpart in the reference (its flash is a serial loop over cells,
