@@ -279,15 +279,18 @@ int  ocpx_get_flash_mapping(ocpx_ctx* ctx, int32_t* lanes_per_cell);
  * (0 where one does not apply), then the chosen one in natural order.
  * Synchronous. */
 int  ocpx_autotune_flash(ocpx_ctx* ctx, double* ms);
-/* Stages of the batched flash.  No counterpart in the reference; same
- * results either way.  1 (default): in a context of one model, the strict
- * stability test (StableSSM + StableNR, src/OCPPhaseEquilibrium.cpp:397-521)
- * runs in its own stages -- the SSM loops of every cell's trial compositions
- * in a lean kernel, the Newton steps of the trials that need one in a
- * second, then per cell the first unstable trial in the reference's order --
- * and the mapped kernel of ocpx_set_flash_mapping takes only the cells left
- * (splits, flash types 1 and 2, the relaxed test); 0: the mapped kernel takes
- * every cell.  Contexts of several models always run unstaged. */
+/* Stages of the batched flash.  No counterpart in the reference; the same
+ * algorithm either way (sums may round differently).  In a context of one
+ * model, 1: the strict stability test (StableSSM + StableNR,
+ * src/OCPPhaseEquilibrium.cpp:397-521) runs in its own stages -- the SSM
+ * loops of every cell's trial compositions in a lean kernel, the Newton steps
+ * of the trials that need one in a second, then per cell the first unstable
+ * trial in the reference's order -- and the mapped kernel of
+ * ocpx_set_flash_mapping takes only the cells left (splits, flash types 1
+ * and 2, the relaxed test); 2 (default): as 1, and the first SplitSSM2 pass
+ * of the unstable cells (:586-649) runs in a lean kernel of its own, the
+ * mapped kernel resuming them at SplitNR (:807-823); 0: the mapped kernel
+ * takes every cell.  Contexts of several models always run unstaged. */
 int  ocpx_set_flash_stages(ocpx_ctx* ctx, int on);
 int  ocpx_get_flash_stages(ocpx_ctx* ctx, int32_t* on);
 /* Queue order of the batched flash.  No counterpart in the reference (its
@@ -382,10 +385,11 @@ double ocpx_xi_phase(const ocpx_fluid* fluid, double P, double T, const double* 
 /* ≙ LinearSystem::AssembleMatLinearSolver + Solve (src/LinearSystem.cpp:96-119)
  * as IsoT_FIM::SolveLinearSystem calls them (src/IsoThermalMethod.cpp:754-
  * 768), on the device BSR of ocpx_assemble: J u = rhs (rhs = -R, CopyRhs).
- * BiCGStab with a block-Jacobi preconditioner (inverse of each diagonal
- * block) in place of the reference's external FASP/PETSc solvers; stops at
- * ||r|| / ||b|| < rtol (the reference's stop_type 1; its decks use 1e-3 and
- * 200 iterations, data/spe5/bsr.fasp).  *iters = iterations, negative when
+ * The method is ocpx_set_linear_solver's (default: ABF decoupling + FGMRES(30)
+ * with a CPR-type preconditioner, as the reference's FASP set-up) in place
+ * of the reference's external FASP/PETSc solvers; stops at ||r|| / ||b|| <
+ * rtol (the reference's stop_type 1, on the decoupled system as FASP does;
+ * its decks use 1e-3 and 200 iterations, data/spe5/bsr.fasp).  *iters = iterations, negative when
  * not converged (the reference's convention); *relres = final ||r||/||b||.
  * u (host [nb*(nc+1)], may be NULL) receives the last iterate; only a
  * converged solution stays on the device for ocpx_get_solution(ctx, NULL,
@@ -398,6 +402,16 @@ double ocpx_xi_phase(const ocpx_fluid* fluid, double P, double T, const double* 
  * adds each well row's entry 0 to its bhp (GetSolutionFIM).  Synchronous. */
 int  ocpx_linear_solve(ocpx_ctx* ctx, double rtol, int32_t maxit, double* u, int32_t* iters,
                        double* relres);
+/* Method of ocpx_linear_solve.  1 (default), as the reference's FASP set-up
+ * (data/spe5/bsr.fasp: decoup_type 1, solver_type 6, restart 30): the system
+ * is decoupled by ABF (each block row times the inverse of its diagonal
+ * block, src/Decoupling.cpp:23-57) and solved by FGMRES(30), right-
+ * preconditioned by a CPR-type two-stage preconditioner -- one V-cycle of
+ * an aggregation AMG on the decoupled pressure block, then one multicolour
+ * block Gauss-Seidel sweep -- stopping on the decoupled ||r|| / ||b||.
+ * 0: BiCGStab with block Jacobi on the system as assembled. */
+int  ocpx_set_linear_solver(ocpx_ctx* ctx, int method);
+int  ocpx_get_linear_solver(ocpx_ctx* ctx, int32_t* method);
 
 /* ≙ IsoT_FIM::UpdateLastTimeStep (src/IsoThermalMethod.cpp:1473-1515) with
  * SkipPSAVarset::UpdateLastTimeStep (src/AcceleratePEC.cpp:24-31): keeps a

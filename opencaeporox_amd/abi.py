@@ -133,6 +133,8 @@ def load(path=None):
         "ocpx_set_flash_queue_order": (C.c_int, [vp, C.c_int]),
         "ocpx_get_flash_queue_order": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_set_flash_stages": (C.c_int, [vp, C.c_int]),
+        "ocpx_set_linear_solver": (C.c_int, [vp, C.c_int]),
+        "ocpx_get_linear_solver": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_phase_equilibrium_noalloc": (C.c_int, [C.POINTER(Fluid), i64, vp, vp, vp, vp, vp, vp, vp, vp]),
         "ocpx_get_flash_stages": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_timing_enable": (C.c_int, [vp, C.c_int]),

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT_DIR = os.path.join(PKG, "lib")
 OBJ_DIR = os.path.join(PKG, "lib", "obj")
-SOURCES = ["pe.hip", "pe_team.hip", "pe_sta.hip", "tail.hip", "fim.hip", "solver.hip", "wells.hip", "noalloc.hip", "ocpx_host.cpp"]
+SOURCES = ["pe.hip", "pe_team.hip", "pe_sta.hip", "tail.hip", "fim.hip", "solver.hip", "wells.hip", "noalloc.hip", "cpr.hip", "ocpx_host.cpp"]
 ARCH = os.environ.get("OCPX_ARCH", "gfx950")
 
 

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -17,6 +18,7 @@
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 #include "layout.hpp"
+#include "cpr.hpp"
 
 namespace ocpx {
 int64_t res_asm_blocks(int nc, int64_t nb);
@@ -88,8 +90,9 @@ struct ocpx_ctx {
     int pe_lanes_auto = 0;       // the autotuner's choice, 0 if it has not run
     int team_blocks[3] = {0, 0, 0}; // persistent grid of the team kernels, T = 4 / 8 / 16
     int ncu = 256;               // compute units of the device
-    int sta_grid[2] = {0, 0};    // persistent grids of the stability stages (SSM, Newton)
-    int stages = 1;              // batched flash: strict stability test in its own stages (pe_sta.hip)
+    int sta_grid[3] = {0, 0, 0}; // persistent grids of the flash stages (stability SSM, Newton, split)
+    int stages = 2;              // batched flash: 1 strict stability test in its own stages, 2 and
+                                 // the split's first SSM pass (pe_sta.hip), 0 none
     bool sta_ready = false;      // the stage buffers are allocated
     StaIO sta{};
     // Peaceman wells (ocpx_set_wells)
@@ -109,6 +112,15 @@ struct ocpx_ctx {
     bool res_current = false;    // residual / Jacobian were formed from the current state
     int32_t solved_wells = 0;    // well rows in the last ocpx_linear_solve
     int64_t sol_cap = 0;         // length of the solver vectors
+    // FGMRES(30) with the CPR-type preconditioner (cpr.hpp)
+    int lsolver = 1;             // 1 FGMRES + CPR, 0 BiCGStab + block Jacobi
+    std::unique_ptr<PressureAmg> amg;   // built at the first solve from the connection graph
+    std::vector<int32_t> gs_cptr;        // colour lists of the block Gauss-Seidel stage
+    int32_t* gs_rows = nullptr;
+    double *fg_V = nullptr, *fg_Z = nullptr, *fg_w = nullptr, *fg_t = nullptr, *fg_x1 = nullptr;
+    double *fg_r2 = nullptr, *fg_pb = nullptr, *fg_px = nullptr, *fg_h = nullptr, *fg_part = nullptr;
+    double* fg_out = nullptr;
+    int64_t fg_cap = 0;
     // timing
     bool timing = false;
     TimingAcc tacc[9];
@@ -265,18 +277,21 @@ int sta_alloc(ocpx_ctx* c)
     StaIO& s = c->sta;
     unsigned* cnt = nullptr;
     int rc;
-    if ((rc = dalloc(c, &cnt, 4)) || (rc = dalloc(c, &s.glist, n)) || (rc = dalloc(c, &s.route, n)) ||
+    if ((rc = dalloc(c, &cnt, 8)) || (rc = dalloc(c, &s.glist, n)) || (rc = dalloc(c, &s.route, n)) ||
         (rc = dalloc(c, &s.decided, n)) || (rc = dalloc(c, &s.defer, n)) || (rc = dalloc(c, &s.Ytd, n)) ||
         (rc = dalloc(c, &s.evals, n)) || (rc = dalloc(c, &s.its, 4 * n)) || (rc = dalloc(c, &s.item, 4 * n)) ||
         (rc = dalloc(c, &s.di, (size_t)c->NC * n)) || (rc = dalloc(c, &s.nr_cell, cap)) ||
         (rc = dalloc(c, &s.nr_k, cap)) || (rc = dalloc(c, &s.nr_flag, cap)) || (rc = dalloc(c, &s.nr_its, cap)) ||
         (rc = dalloc(c, &s.nr_evals, cap)) || (rc = dalloc(c, &s.nr_Yt, cap)) ||
         (rc = dalloc(c, &s.nr_y, (size_t)c->NC * cap)) || (rc = dalloc(c, &s.rs_Yt, n)) ||
-        (rc = dalloc(c, &s.rs_cnt, 4 * n)))
+        (rc = dalloc(c, &s.rs_cnt, 8 * n)) || (rc = dalloc(c, &s.slist, n)) ||
+        (rc = dalloc(c, &s.rs_v, (size_t)5 * c->NC * n)) || (rc = dalloc(c, &s.rs_nu, n)))
         return rc;
     s.qhead = cnt;
     s.gcount = cnt + 1;
     s.ncount = cnt + 2;
+    s.scount = cnt + 3;
+    s.shead = cnt + 4;
     s.nr_cap = cap;
     c->sta_ready = true;
     return OCPX_OK;
@@ -312,12 +327,15 @@ int launch_pe(ocpx_ctx* c, int mode)
     {
         TimerScope t(c, 0);
         if (staged) {
-            HIPCHK(c, run_pe_sta(c->NC, c->df, io, c->sta, c->sta_grid[0], c->sta_grid[1], c->st));
+            c->sta.split = c->stages >= 2;
+            HIPCHK(c, run_pe_sta(c->NC, c->df, io, c->sta, c->sta_grid, c->st));
             io.qorder = c->sta.glist;
             io.nq = c->sta.gcount;
             io.sta_route = c->sta.route;
             io.rs_Yt = c->sta.rs_Yt;
             io.rs_cnt = c->sta.rs_cnt;
+            io.rs_v = c->sta.rs_v;
+            io.rs_nu = c->sta.rs_nu;
         }
         if (mode == OCPX_FLASH_BATCHED && c->pe_lanes > 1) {
             const int T = c->pe_lanes;
@@ -602,7 +620,7 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     c->ncu = ncu > 0 ? ncu : 256;
-    for (int q = 0; q < 2; q++) c->sta_grid[q] = c->ncu * pe_sta_blocks_per_cu(c->NC, q);
+    for (int q = 0; q < 3; q++) c->sta_grid[q] = c->ncu * pe_sta_blocks_per_cu(c->NC, q);
     const int per_cu = pe_batched_blocks_per_cu(c->NC);
     c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
     for (int q = 0; q < 3; q++) {
@@ -826,9 +844,23 @@ int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
     return ocpx_set_flash_mapping(c, best);
 }
 
+int ocpx_set_linear_solver(ocpx_ctx* c, int method)
+{
+    if (!c || (method != 0 && method != 1)) return OCPX_E_ARG;
+    c->lsolver = method;
+    return OCPX_OK;
+}
+
+int ocpx_get_linear_solver(ocpx_ctx* c, int32_t* method)
+{
+    if (!c || !method) return OCPX_E_ARG;
+    *method = c->lsolver;
+    return OCPX_OK;
+}
+
 int ocpx_set_flash_stages(ocpx_ctx* c, int on)
 {
-    if (!c || (on != 0 && on != 1)) return OCPX_E_ARG;
+    if (!c || on < 0 || on > 2) return OCPX_E_ARG;
     if (on != c->stages) destroy_graph(c);
     c->stages = on;
     return OCPX_OK;
@@ -1218,9 +1250,199 @@ int ocpx_get_solution(ocpx_ctx* c, const double* u, double dSmax)
     return OCPX_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The pressure AMG and the colouring of the block graph, from the bulk
+// pattern and the connections' transmissibilities (once per context and
+// transmissibility set).
+int cpr_setup(ocpx_ctx* c)
+{
+    if (c->amg && c->amg->ready()) return OCPX_OK;
+    const int64_t nb = c->nb;
+    std::vector<int64_t> rp(nb + 1);
+    std::vector<int32_t> col(c->nnzb), nbc(c->nnzb);
+    std::vector<double> trans(c->nconn > 0 ? c->nconn : 1), w(c->nnzb, 0.0);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(rp.data(), c->rowptr, (nb + 1) * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(col.data(), c->colind, c->nnzb * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(nbc.data(), c->nbr_conn, c->nnzb * 4, hipMemcpyDeviceToHost));
+    if (c->nconn) HIPCHK(c, hipMemcpy(trans.data(), c->trans, c->nconn * 8, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < c->nnzb; k++)
+        if (nbc[k] >= 0) w[k] = std::fabs(trans[nbc[k]]);
+    auto amg = std::make_unique<PressureAmg>();
+    std::string err;
+    if (amg->setup(nb, rp, col, w, err)) return fail(c, OCPX_E_NOMEM, "pressure AMG set-up: " + err);
+    c->amg = std::move(amg);
+    if (!c->gs_rows) {
+        std::vector<int32_t> rows;
+        color_rows(nb, rp, col, c->gs_cptr, rows);
+        int rc = dalloc(c, &c->gs_rows, rows.size());
+        if (rc) return rc;
+        HIPCHK(c, hipMemcpy(c->gs_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+    }
+    return OCPX_OK;
+}
+
+// FGMRES(30) (Saad, SIAM J. Sci. Comput. 14 (1993) 461-469) on the
+// ABF-decoupled system D^{-1} J x = D^{-1} rhs (D the diagonal blocks, the
+// well rows' too; src/Decoupling.cpp:23-57), right-preconditioned by two
+// stages: the pressure AMG V-cycle on the decoupled pressure block, then one
+// multicolour block Gauss-Seidel sweep on the remaining residual.  Classical
+// Gram-Schmidt with one reorthogonalisation; stops on the decoupled
+// ||r|| / ||b||, as FASP's stop_type 1 on the decoupled system.  x0 = 0,
+// the solution in c->sol.
+int fgmres_cpr(ocpx_ctx* c, const double* rhs, int64_t len, int32_t nwj, double rtol, int32_t maxit, int& it,
+               double& res, bool& conv)
+{
+    constexpr int M = 30;
+    const int B = c->nc + 1;
+    const int64_t nb = c->nb;
+    hipStream_t st = c->st;
+    int rc = cpr_setup(c);
+    if (rc) return rc;
+    if (c->fg_cap < len) {
+        for (double** q : {&c->fg_V, &c->fg_Z, &c->fg_w, &c->fg_t, &c->fg_x1, &c->fg_r2}) { dfree(c, *q); *q = nullptr; }
+        if ((rc = dalloc(c, &c->fg_V, (size_t)(M + 1) * len)) || (rc = dalloc(c, &c->fg_Z, (size_t)M * len)) ||
+            (rc = dalloc(c, &c->fg_w, (size_t)len)) || (rc = dalloc(c, &c->fg_t, (size_t)len)) ||
+            (rc = dalloc(c, &c->fg_x1, (size_t)len)) || (rc = dalloc(c, &c->fg_r2, (size_t)len)))
+            return rc;
+        c->fg_cap = len;
+    }
+    if (!c->fg_pb) {
+        if ((rc = dalloc(c, &c->fg_pb, (size_t)nb)) || (rc = dalloc(c, &c->fg_px, (size_t)nb)) ||
+            (rc = dalloc(c, &c->fg_h, 64)) || (rc = dalloc(c, &c->fg_part, (size_t)mdot_scratch_len())) ||
+            (rc = dalloc(c, &c->fg_out, 64)))
+            return rc;
+    }
+    const BsrIO io = bsr_io(c);
+    const WellIO wio = well_io(c);
+    double *V = c->fg_V, *Z = c->fg_Z, *w = c->fg_w, *t = c->fg_t, *x = c->sol;
+    // y = D^{-1} J x
+    auto aop = [&](const double* in, double* out) -> int {
+        {
+            TimerScope ts(c, 8);
+            HIPCHK(c, run_bsr_spmv(B, io.rowptr, io.colind, io.val, in, t, nb, st));
+        }
+        if (nwj) HIPCHK(c, run_well_spmv(wio, B, nb, in, t, st));
+        HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv, t, out, nb, st));
+        if (nwj) HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv_w, t + nb * B, out + nb * B, nwj, st));
+        return OCPX_OK;
+    };
+    // z = M^{-1} v
+    auto precond = [&](const double* v, double* z) -> int {
+        HIPCHK(c, run_gather_p(B, v, c->fg_pb, nb, st));
+        HIPCHK(c, c->amg->vcycle(c->fg_pb, c->fg_px, st));
+        HIPCHK(c, run_scatter_p(B, c->fg_px, c->fg_x1, nb, len, st));
+        int rr = aop(c->fg_x1, c->fg_r2);
+        if (rr) return rr;
+        HIPCHK(c, run_axpbypcz(c->fg_r2, v, -1.0, c->fg_r2, 0, nullptr, len, st));    // r2 = v - A x1
+        // z = r2 on every row, then the colours of the bulk rows in turn
+        // (rows of one colour read only rows of other colours)
+        HIPCHK(c, run_axpbypcz(z, nullptr, 0, nullptr, 0, nullptr, nb * B, st));
+        if (nwj) HIPCHK(c, run_axpbypcz(z + nb * B, c->fg_r2 + nb * B, 0, nullptr, 0, nullptr, (int64_t)nwj * B, st));
+        for (size_t k = 0; k + 1 < c->gs_cptr.size(); k++)
+            HIPCHK(c, run_gs_color(B, c->gs_rows + c->gs_cptr[k], c->gs_cptr[k + 1] - c->gs_cptr[k], io.rowptr,
+                                   io.colind, io.val, c->dinv, c->fg_r2, z, st));
+        HIPCHK(c, run_axpbypcz(z, z, 1.0, c->fg_x1, 0, nullptr, len, st));
+        return OCPX_OK;
+    };
+    auto dots = [&](const double* Vb, int m, const double* vec, double* out) -> int {
+        HIPCHK(c, run_mdot(Vb, len, m, vec, len, c->fg_part, c->fg_out, st));
+        HIPCHK(c, hipMemcpyAsync(out, c->fg_out, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        return OCPX_OK;
+    };
+    HIPCHK(c, c->amg->values(B, io.val, c->dinv, st));
+    // b = D^{-1} rhs into V_0's slot for now; x = 0
+    double* bh = c->fg_r2;
+    HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv, rhs, V, nb, st));
+    if (nwj) HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv_w, rhs + nb * B, V + nb * B, nwj, st));
+    HIPCHK(c, run_axpbypcz(x, nullptr, 0, nullptr, 0, nullptr, len, st));
+    double bb = 0;
+    if ((rc = dots(V, 1, V, &bb))) return rc;
+    const double bnorm = std::sqrt(bb);
+    double beta = bnorm;
+    it = 0;
+    res = 0;
+    conv = bnorm == 0;
+    std::vector<double> H((M + 1) * M), cs(M), sn(M), g(M + 1), y(M), h(M + 1), h2(M + 1);
+    while (!conv && it < maxit && std::isfinite(beta) && beta > 0) {
+        HIPCHK(c, run_axpbypcz(V, nullptr, 1.0 / beta, V, 0, nullptr, len, st));        // V_0 = r / beta
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        bool stop = false;
+        for (int j = 0; j < M && it < maxit; j++) {
+            double* Vj = V + (size_t)j * len;
+            double* Zj = Z + (size_t)j * len;
+            if ((rc = precond(Vj, Zj)) || (rc = aop(Zj, w))) return rc;
+            if ((rc = dots(V, j + 1, w, h.data()))) return rc;
+            for (int i = 0; i <= j; i++) h2[i] = -h[i];
+            HIPCHK(c, hipMemcpyAsync(c->fg_h, h2.data(), (size_t)(j + 1) * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(c, run_maxpy(w, V, len, j + 1, c->fg_h, len, st));
+            if ((rc = dots(V, j + 1, w, h2.data()))) return rc;
+            for (int i = 0; i <= j; i++) { h[i] += h2[i]; h2[i] = -h2[i]; }
+            HIPCHK(c, hipMemcpyAsync(c->fg_h, h2.data(), (size_t)(j + 1) * 8, hipMemcpyHostToDevice, st));
+            HIPCHK(c, run_maxpy(w, V, len, j + 1, c->fg_h, len, st));
+            double ww = 0;
+            HIPCHK(c, run_dot(w, w, len, c->dpart, c->dout, st));
+            HIPCHK(c, hipMemcpyAsync(&ww, c->dout, sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipStreamSynchronize(st));
+            const double hn = std::sqrt(ww);
+            for (int i = 0; i <= j; i++) H[i * M + j] = h[i];
+            H[(j + 1) * M + j] = hn;
+            for (int i = 0; i < j; i++) {             // earlier rotations
+                const double a = H[i * M + j], b2 = H[(i + 1) * M + j];
+                H[i * M + j] = cs[i] * a + sn[i] * b2;
+                H[(i + 1) * M + j] = -sn[i] * a + cs[i] * b2;
+            }
+            const double a = H[j * M + j], b2 = H[(j + 1) * M + j];
+            const double den = std::hypot(a, b2);
+            cs[j] = den > 0 ? a / den : 1.0;
+            sn[j] = den > 0 ? b2 / den : 0.0;
+            H[j * M + j] = den;
+            H[(j + 1) * M + j] = 0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            it++;
+            k = j + 1;
+            res = std::fabs(g[j + 1]) / bnorm;
+            if (!std::isfinite(res)) return fail(c, OCPX_E_HIP, "FGMRES: non-finite residual");
+            if (res < rtol || hn == 0) { stop = true; break; }
+            HIPCHK(c, run_axpbypcz(V + (size_t)(j + 1) * len, nullptr, 1.0 / hn, w, 0, nullptr, len, st));
+        }
+        for (int i = k - 1; i >= 0; i--) {           // H y = g, upper triangular
+            double v = g[i];
+            for (int q = i + 1; q < k; q++) v -= H[i * M + q] * y[q];
+            y[i] = H[i * M + i] != 0 ? v / H[i * M + i] : 0.0;
+        }
+        HIPCHK(c, hipMemcpyAsync(c->fg_h, y.data(), (size_t)k * 8, hipMemcpyHostToDevice, st));
+        HIPCHK(c, run_maxpy(x, Z, len, k, c->fg_h, len, st));
+        // the true decoupled residual: V_0 = D^{-1} rhs - D^{-1} J x
+        if ((rc = aop(x, w))) return rc;
+        HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv, rhs, bh, nb, st));
+        if (nwj) HIPCHK(c, run_bsr_spmv(B, nullptr, nullptr, c->dinv_w, rhs + nb * B, bh + nb * B, nwj, st));
+        HIPCHK(c, run_axpbypcz(V, bh, -1.0, w, 0, nullptr, len, st));
+        double rr2 = 0;
+        if ((rc = dots(V, 1, V, &rr2))) return rc;
+        beta = std::sqrt(rr2);
+        res = beta / bnorm;
+        if (res < rtol) conv = true;
+        if (stop && !conv && k == 0) break;
+    }
+    return OCPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 // BiCGStab, right-preconditioned with block Jacobi, x0 = 0, stopping on
 // ||r|| / ||b|| (the reference's stop_type 1); scalars come back to the host
-// once per dot product.
+// once per dot product.  Or (the default) FGMRES with the CPR-type
+// preconditioner, fgmres_cpr.
 int ocpx_linear_solve(ocpx_ctx* c, double rtol, int32_t maxit, double* u, int32_t* iters, double* relres)
 {
     if (!c || !(rtol > 0) || maxit <= 0) return OCPX_E_ARG;
@@ -1274,6 +1496,21 @@ int ocpx_linear_solve(ocpx_ctx* c, double rtol, int32_t maxit, double* u, int32_
         rhs = c->rhsf;
     }
     HIPCHK(c, run_bsr_diag_inv(B, io.rowptr, io.val, nb, c->dinv, st));
+    if (c->lsolver == 1) {
+        int it = 0;
+        double res = 0;
+        bool conv = false;
+        if ((rc = fgmres_cpr(c, rhs, len, nwj, rtol, maxit, it, res, conv))) return rc;
+        if (iters) *iters = conv ? it : -std::max(it, 1);
+        if (relres) *relres = res;
+        if (u) {
+            HIPCHK(c, hipMemcpyAsync(u, x, (size_t)len * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipStreamSynchronize(st));
+        }
+        c->solved = conv;
+        c->solved_wells = conv ? nwj : 0;
+        return OCPX_OK;
+    }
     HIPCHK(c, run_axpbypcz(x, nullptr, 0, nullptr, 0, nullptr, len, st));
     HIPCHK(c, run_axpbypcz(r, rhs, 0, nullptr, 0, nullptr, len, st));
     HIPCHK(c, run_axpbypcz(rh, rhs, 0, nullptr, 0, nullptr, len, st));
@@ -1383,6 +1620,7 @@ int ocpx_set_geometry(ocpx_ctx* c, const double* v, const double* trans)
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (v) HIPCHK(c, hipMemcpy(c->v, v, (size_t)c->nb * 8, hipMemcpyHostToDevice));
     if (trans && c->nconn) HIPCHK(c, hipMemcpy(c->trans, trans, (size_t)c->nconn * 8, hipMemcpyHostToDevice));
+    if (trans) c->amg.reset();          // the pressure AMG's aggregates follow the transmissibilities
     c->aux_valid = false;
     c->res_current = false;
     return OCPX_OK;

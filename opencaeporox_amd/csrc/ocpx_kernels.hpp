@@ -37,11 +37,14 @@ struct PeIO {
     const int32_t* qorder;      // [nb] cell handed out at each queue position (batched mode), or null
     const int32_t* mid;         // [nb] model of each cell (multi-model contexts), or null
     const unsigned* nq;         // queue length on the device (batched mode), or null (nb)
-    // cells the stability stages found unstable (route 2) start at the split
-    // with the deciding trial's Yt and counters (pe_sta.hip), or null
+    // cells the stability stages found unstable start at the split (route 2)
+    // with the deciding trial's Yt and counters, or at SplitNR (route 3) from
+    // what the split stage left (pe_sta.hip); null: no stages
     const int32_t* sta_route;   // [nb]
     const double* rs_Yt;        // [nb]
-    const int32_t* rs_cnt;      // [4][nb]: ITERS rows 0, 1, 5 and 6 (= row 7)
+    const int32_t* rs_cnt;      // [8][nb]: ITERS rows 0, 1, 5, 6 (= row 7), 2, 4; SSM converged
+    const double* rs_v;         // [5][Nc][nb]: x0, x1, K, ln phi(x0), ln phi(x1)
+    const double* rs_nu;        // [nb]
 };
 
 // device fluid constants of one PVT region (ocpx_host.cpp)
@@ -51,11 +54,16 @@ int pe_batched_blocks_per_cu(int NC);
 
 // stability stages of the batched flash (pe_sta.hip)
 struct StaIO {
-    unsigned* qhead;            // counters, three consecutive words: stage-1 queue head,
+    unsigned* qhead;            // counters, five consecutive words: stage-1 queue head,
     unsigned* gcount;           //   cells for the general kernel,
-    unsigned* ncount;           //   queued Newton trials
+    unsigned* ncount;           //   queued Newton trials,
+    unsigned* scount;           //   cells for the split stage,
+    unsigned* shead;            //   split-stage queue head
     int32_t* glist;             // [nb] cells for the general kernel
-    int32_t* route;             // [nb] 0 general kernel, 1 stable/undecided by stage 1, 2 split
+    int32_t* slist;             // [nb] cells for the split stage
+    int split;                  // 1: unstable cells take the split stage before the general kernel
+    int32_t* route;             // [nb] 0 general kernel, 1 stable/undecided by stage 1, 2 unstable,
+                                //      3 resumed at SplitNR after the split stage
     int32_t* decided;           // [nb] first trial stage 1 found unstable, 4 none
     int32_t* defer;             // [nb] bit k: trial k queued for its Newton step
     double* Ytd;                // [nb] Yt of the decided trial
@@ -67,13 +75,17 @@ struct StaIO {
     int32_t *nr_cell, *nr_k, *nr_flag, *nr_its, *nr_evals;   // [nr_cap]
     double *nr_Yt, *nr_y;       // [nr_cap], [Nc][nr_cap]
     double* rs_Yt;              // [nb]
-    int32_t* rs_cnt;            // [4][nb]
+    int32_t* rs_cnt;            // [8][nb] counters of the stages: SSMsta, NRsta, evaluations,
+                                //   NR steps, SSMsp, Rachford-Rice, SSM converged
+    double* rs_v;               // [5][Nc][nb] after the split stage: x0, x1, K, ln phi(x0), ln phi(x1)
+    double* rs_nu;              // [nb] nu of the split stage's last Rachford-Rice solve
 };
-// stages 1-3 of the batched flash for one model on persistent grids of grid1
-// (SSM loops) and grid2 (Newton steps) 64-lane blocks
-hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int grid1, int grid2,
-                      hipStream_t st);
-// resident blocks per compute unit of the SSM (which = 0) and Newton (1) stages
+// stages of the batched flash for one model on persistent grids of grid[0]
+// (stability SSM loops), grid[1] (stability Newton steps) and grid[2] (split
+// SSM loops) 64-lane blocks
+hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st);
+// resident blocks per compute unit of the stability SSM (which = 0), Newton
+// (1) and split (2) stages
 int pe_sta_blocks_per_cu(int NC, int which);
 
 // queue order for the next batched flash (pe.hip): cells by descending count

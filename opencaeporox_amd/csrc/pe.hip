@@ -321,6 +321,30 @@ OCPX_HD void pe_begin(const DevFluid& f, const PeIO& io, int cell, PeLane<NC>& L
         split_start(f, L, io, false);
         return;
     }
+    if (io.sta_route && io.sta_route[cell] == 3) {   // SplitSSM2's first pass done by the split stage
+        L.cnt[0] = io.rs_cnt[cell];
+        L.cnt[1] = io.rs_cnt[n + cell];
+        L.cnt[5] = io.rs_cnt[2 * n + cell];
+        L.cnt[6] = L.cnt[7] = io.rs_cnt[3 * n + cell];
+        L.cnt[2] = io.rs_cnt[4 * n + cell];
+        L.cnt[4] = io.rs_cnt[5 * n + cell];
+        L.ssm_conv = io.rs_cnt[6 * n + cell];
+        L.Yt = io.rs_Yt[cell];
+        L.NP = 2;
+        L.stage = 0;
+        L.restarts = 0;
+        L.nu0 = io.rs_nu[cell];
+        double l1[NC];
+        UNROLL for (int i = 0; i < NC; i++) {
+            L.a2[i] = io.rs_v[(size_t)i * n + cell];
+            L.a3[i] = io.rs_v[(size_t)(NC + i) * n + cell];
+            L.a4[i] = io.rs_v[(size_t)(2 * NC + i) * n + cell];
+            L.a5[i] = io.rs_v[(size_t)(3 * NC + i) * n + cell];
+            l1[i] = io.rs_v[(size_t)(4 * NC + i) * n + cell];
+        }
+        split_nr_begin(f, L, io, l1);
+        return;
+    }
     UNROLL for (int i = 0; i < NC; i++) L.y[i] = L.z[i];
     L.st = ST_FEED;
 }

@@ -21,8 +21,14 @@
 //   stage 3  pe_sta_join  per cell, the first trial in order that is unstable
 //                         (the reference's early exit), the counters of the
 //                         trials up to it; stable cells are written out,
-//                         unstable ones queued for the split in the general
-//                         kernel, which resumes them with that trial's Yt.
+//                         unstable ones queued for the split with that
+//                         trial's Yt.
+//   stage 4  pe_sta_split persistent lanes, one unstable cell each:
+//                         SplitSSM2's first pass (Wilson K, RachfordRice2,
+//                         UpdateXRR, ln phi of both phases, K update) to
+//                         SSMSP's tolerance or cap.  The general kernel
+//                         resumes every such cell at SplitNR and carries on
+//                         (Newton steps, CheckSplit, the restart loop).
 //
 // Trials run past an undecided one are speculative: their results are not
 // used when the earlier trial turns out unstable, exactly as the reference
@@ -37,6 +43,9 @@ namespace sta {
 constexpr int NONE = 4;          // decided: no trial found unstable
 #ifndef STA_WAVES
 #define STA_WAVES 2               // stage-1 wavefronts per SIMD the register budget targets
+#endif
+#ifndef SPLIT_WAVES
+#define SPLIT_WAVES 2             // the same for stage 4
 #endif
 #define DEV __device__ __forceinline__
 
@@ -399,44 +408,172 @@ __global__ void __launch_bounds__(256) pe_sta_join(const DevFluid* __restrict__ 
             split = true;
         }
     }
-    append(split, s.gcount, s.glist, (int32_t)cell);
+    if (s.split) append(split, s.scount, s.slist, (int32_t)cell);
+    else append(split, s.gcount, s.glist, (int32_t)cell);
+}
+
+// stage 4: the first pass of SplitSSM2 (:586-649) for a cell stage 3 found
+// unstable.  Such a cell had one phase before (strict test, flash type 0), so
+// K starts from Wilson's (:598-603) and the pass takes SSMSP's tolerance and
+// cap, both tightened when Yt < 1.1 (:611-614).  Each iteration:
+// RachfordRice2 (:651-705) for nu, UpdateXRR (:778-791) for x1 = z / (1 +
+// nu (K - 1)) and x0 = K x1, ln phi of x0 and of x1, the residual
+// sum (f1 / f0 - 1)^2 in the literal form of :629-635, and K = phi1 / phi0.
+// x0, x1 are formed again from z, K and nu wherever they are needed, so a
+// lane carries z, K, sqrt(Ai) and ln phi(x0) across the iteration.
+template <int NC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPLIT_WAVES)))
+pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
+{
+    const size_t n = io.nb;
+    const int lane = __lane_id();
+    const unsigned total = *s.scount;
+    int cell = -1, iter = 0, maxIt = 0, rr = 0;
+    bool busy = false, drained = false;
+    double P = 0, PT = 0, tol2 = 0, nu = 0;
+    double z[NC], K[NC], sA[NC];
+    while (true) {
+        while (true) {
+            const bool need = !busy && !drained;
+            const unsigned long long m = __ballot(need);
+            if (!m) break;
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(s.shead, (unsigned)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                if (idx < total) {
+                    const DevFluid& f0 = *opaque(fp);
+                    cell = s.slist[idx];
+                    P = io.P[cell];
+                    const double Tk = io.T[cell] + CONV4, logP = log(P);
+                    PT = P / Tk;
+                    double Nt = 0;
+                    UNROLL for (int i = 0; i < NC; i++) { z[i] = io.Ni[(size_t)i * n + cell]; Nt += z[i]; }
+                    UNROLL for (int i = 0; i < NC; i++) {
+                        z[i] = z[i] / Nt;
+                        const double Pr = P / f0.Pc[i], Tr = Tk / f0.Tc[i];
+                        const double q = 1 + f0.mw[i] * (1 - sqrt(Tr));
+                        sA[i] = sqrt(f0.OA[i] * Pr / (Tr * Tr) * (q * q));
+                        K[i] = exp(f0.logPc[i] - logP + f0.kwA[i] * (1 - f0.Tc[i] / Tk));
+                    }
+                    tol2 = f0.SSMsp_tol2;
+                    maxIt = f0.SSMsp_maxIt;
+                    if (s.rs_Yt[cell] < 1.1) {
+                        tol2 *= 1E-1;
+                        maxIt *= 2;
+                    }
+                    iter = 0;
+                    rr = 0;
+                    busy = true;
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__any(busy)) break;
+        if (!busy) continue;
+
+        const DevFluid& f = *opaque(fp);
+        {   // RachfordRice2 on K
+            double Kmin = K[0], Kmax = K[0];
+            UNROLL for (int i = 1; i < NC; i++) { Kmin = fmin(Kmin, K[i]); Kmax = fmax(Kmax, K[i]); }
+            const double numin = 1 / (1 - Kmax), numax = 1 / (1 - Kmin);
+            nu = 0.5 * (numin + numax);
+            int it = 0;
+            while (true) {
+                double rj = 0, J = 0;
+                UNROLL for (int i = 0; i < NC; i++) {
+                    const double km = K[i] - 1;
+                    const double t = 1 / (1 + nu * km);
+                    rj += z[i] * km * t;
+                    J -= z[i] * km * km * t * t;
+                }
+                if (fabs(rj) < f.RR_tol || it > f.RR_maxIt) break;
+                const double d = -rj / J, tn = nu + d;
+                if (tn < numax && tn > numin) nu = tn;
+                else nu = d > 0 ? (nu + numax) / 2 : (nu + numin) / 2;
+                it++;
+            }
+            rr += it;
+        }
+        double x[NC], l0[NC], l1[NC];
+        UNROLL for (int i = 0; i < NC; i++) x[i] = K[i] * (z[i] / (1 + nu * (K[i] - 1)));     // x0
+        lean_lnphi<NC>(f, sA, PT, x, l0);
+        UNROLL for (int i = 0; i < NC; i++) x[i] = z[i] / (1 + nu * (K[i] - 1));              // x1
+        lean_lnphi<NC>(f, sA, PT, x, l1);
+        iter++;
+        double Se = 0;
+        UNROLL for (int i = 0; i < NC; i++) {
+            const double x1 = z[i] / (1 + nu * (K[i] - 1)), x0 = K[i] * x1;
+            const double phi0 = exp(l0[i]), phi1 = exp(l1[i]);
+            const double q = (phi1 * x1 * P) / (phi0 * x0 * P) - 1;
+            Se += q * q;
+        }
+        bool last, conv = true;
+        if (iter > maxIt) { conv = false; last = true; }
+        else last = !(Se > tol2);
+        if (last) {     // x0, x1 of this iteration's K, then the updated K
+            UNROLL for (int i = 0; i < NC; i++) {
+                const double x1 = z[i] / (1 + nu * (K[i] - 1)), x0 = K[i] * x1;
+                s.rs_v[(size_t)(0 * NC + i) * n + cell] = x0;
+                s.rs_v[(size_t)(1 * NC + i) * n + cell] = x1;
+                s.rs_v[(size_t)(3 * NC + i) * n + cell] = l0[i];
+                s.rs_v[(size_t)(4 * NC + i) * n + cell] = l1[i];
+            }
+        }
+        UNROLL for (int i = 0; i < NC; i++) K[i] = exp(l1[i]) / exp(l0[i]);
+        if (last) {
+            UNROLL for (int i = 0; i < NC; i++) s.rs_v[(size_t)(2 * NC + i) * n + cell] = K[i];
+            s.rs_nu[cell] = nu;
+            s.rs_cnt[(size_t)2 * n + cell] += 2 * iter;
+            s.rs_cnt[(size_t)4 * n + cell] = iter;
+            s.rs_cnt[(size_t)5 * n + cell] = rr;
+            s.rs_cnt[(size_t)6 * n + cell] = conv ? 1 : 0;
+            s.route[cell] = 3;
+            busy = false;
+        }
+        append(last, s.gcount, s.glist, cell);
+    }
 }
 
 __global__ void __launch_bounds__(64) sta_reset(unsigned* c)
 {
-    if (threadIdx.x < 3) c[threadIdx.x] = 0;
+    if (threadIdx.x < 5) c[threadIdx.x] = 0;
 }
 
 template <int NC>
 int blocks_per_cu(int which)
 {
     int b = 0;
-    const hipError_t e = which == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_ssm<NC>, 64, 0)
-                                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_nr<NC>, 64, 0);
+    const hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_ssm<NC>, 64, 0)
+                         : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_nr<NC>, 64, 0)
+                                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_split<NC>, 64, 0);
     return e == hipSuccess && b > 0 ? b : 2;
 }
 
 template <int NC>
-hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, int grid1, int grid2, hipStream_t st)
+hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st)
 {
     if (io.nb == 0) return hipSuccess;
     hipLaunchKernelGGL(sta_reset, dim3(1), dim3(64), 0, st, s.qhead);
     const long need = (io.nb + 63) / 64;
-    const long g1 = grid1 < need ? grid1 : need;
-    const long g2 = grid2 < need ? grid2 : need;
-    hipLaunchKernelGGL(pe_sta_ssm<NC>, dim3((unsigned)g1), dim3(64), 0, st, f, io, s);
-    hipLaunchKernelGGL(pe_sta_nr<NC>, dim3((unsigned)g2), dim3(64), 0, st, f, io, s);
+    long g[3];
+    for (int q = 0; q < 3; q++) g[q] = grid[q] < need ? grid[q] : need;
+    hipLaunchKernelGGL(pe_sta_ssm<NC>, dim3((unsigned)g[0]), dim3(64), 0, st, f, io, s);
+    hipLaunchKernelGGL(pe_sta_nr<NC>, dim3((unsigned)g[1]), dim3(64), 0, st, f, io, s);
     hipLaunchKernelGGL(pe_sta_join<NC>, dim3((unsigned)((io.nb + 255) / 256)), dim3(256), 0, st, f, io, s);
+    if (s.split) hipLaunchKernelGGL(pe_sta_split<NC>, dim3((unsigned)g[2]), dim3(64), 0, st, f, io, s);
     return hipGetLastError();
 }
 
 }  // namespace sta
 
-hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, int grid1, int grid2,
-                      hipStream_t st)
+hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st)
 {
     switch (NC) {
-#define OCPX_CASE(N) case N: return sta::launch<N>(f, io, s, grid1, grid2, st);
+#define OCPX_CASE(N) case N: return sta::launch<N>(f, io, s, grid, st);
         OCPX_CASE(2) OCPX_CASE(3) OCPX_CASE(4) OCPX_CASE(5) OCPX_CASE(6) OCPX_CASE(7)
         OCPX_CASE(8) OCPX_CASE(9) OCPX_CASE(10) OCPX_CASE(11) OCPX_CASE(12)
 #undef OCPX_CASE

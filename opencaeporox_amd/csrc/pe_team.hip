@@ -578,6 +578,31 @@ DEV void begin(const DevFluid& f, LaneFluid<NC, T>& lf, const PeIO& io, int cell
         split_start(f, lf, C, io, false, ln);
         return;
     }
+    if (io.sta_route && io.sta_route[cell] == 3) {   // SplitSSM2's first pass done by the split stage
+        C.cnt[0] = io.rs_cnt[cell];
+        C.cnt[1] = io.rs_cnt[n + cell];
+        C.cnt[5] = io.rs_cnt[2 * n + cell];
+        C.cnt[6] = C.cnt[7] = io.rs_cnt[3 * n + cell];
+        C.cnt[2] = io.rs_cnt[4 * n + cell];
+        C.cnt[4] = io.rs_cnt[5 * n + cell];
+        C.ssm_conv = io.rs_cnt[6 * n + cell];
+        C.Yt = io.rs_Yt[cell];
+        C.NP = 2;
+        C.stage = 0;
+        C.restarts = 0;
+        C.nu0 = io.rs_nu[cell];
+        double l1[L];
+        SLOTS {
+            const size_t i = ln.act[c] ? (size_t)(c * T + ln.t) : 0;
+            C.a2[c] = ln.act[c] ? io.rs_v[i * n + cell] : 0.0;
+            C.a3[c] = ln.act[c] ? io.rs_v[(NC + i) * n + cell] : 0.0;
+            C.a4[c] = ln.act[c] ? io.rs_v[(2 * NC + i) * n + cell] : 0.0;
+            C.a5[c] = ln.act[c] ? io.rs_v[(3 * NC + i) * n + cell] : 0.0;
+            l1[c] = ln.act[c] ? io.rs_v[(4 * NC + i) * n + cell] : 0.0;
+        }
+        split_nr_begin(f, lf, C, io, l1, ln);
+        return;
+    }
     SLOTS C.y[c] = C.z[c];
     C.st = ST_FEED;
 }

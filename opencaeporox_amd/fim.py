@@ -164,8 +164,10 @@ class IsoTFIM:
         return bool(v.value)
 
     def set_flash_stages(self, on):
-        """Strict stability test of the batched flash in its own stages (default on)."""
-        self._chk(self.lib.ocpx_set_flash_stages(self.h, 1 if on else 0), "flash stages")
+        """Stages of the batched flash: 0 (or False) none, 1 the strict stability
+        test in its own kernels, 2 (default, or True) also the split's first SSM pass."""
+        v = 2 if on is True else (0 if on is False else int(on))
+        self._chk(self.lib.ocpx_set_flash_stages(self.h, v), "flash stages")
 
     def flash_stages(self):
         v = C.c_int32()
@@ -216,9 +218,15 @@ class IsoTFIM:
         u = np.ascontiguousarray(u, np.float64)
         self._chk(self.lib.ocpx_get_solution(self.h, u.ctypes.data, dSmax), "get solution")
 
+    def set_linear_solver(self, method):
+        """1 (default): ABF decoupling + FGMRES(30) with the CPR-type preconditioner;
+        0: BiCGStab + block Jacobi."""
+        self._chk(self.lib.ocpx_set_linear_solver(self.h, int(method)), "linear solver")
+
     def LinearSolve(self, rtol=1e-3, maxit=200, fetch=True):
-        """Device BiCGStab + block Jacobi on the assembled system (ocpx_linear_solve).
-        Returns (u or None, iterations (negative if not converged), ||r||/||b||)."""
+        """Device solve of the assembled system (ocpx_linear_solve, the method of
+        set_linear_solver).  Returns (u or None, iterations (negative if not
+        converged), the final relative residual)."""
         rows = self.nb + getattr(self, "_wells_in_system", 0)     # well rows after ocpx_well_terms
         u = np.zeros(rows * self.bs) if fetch else None
         it, rr = C.c_int32(), C.c_double()

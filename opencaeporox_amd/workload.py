@@ -278,8 +278,13 @@ def cube_grid(seed=1):
     return Workload("10x10x10 cubes, Nc=6", fl, make_sat(), rock, grid, st)
 
 
-def heterogeneous_grid(nx=100, ny=100, nz=50, Nc=6, seed=3, lbc_exact=0):
-    """Hydrostatic P + N(0, 1 MPa), ln kx ~ N(ln 100, 2), kz = 0.1 kx."""
+def heterogeneous_grid(nx=100, ny=100, nz=50, Nc=6, seed=3, lbc_exact=0, composition="per-cell"):
+    """Hydrostatic P + N(0, 1 MPa), ln kx ~ N(ln 100, 2), kz = 0.1 kx.
+    composition "per-cell": an independent Dirichlet(1) draw per cell (every
+    phase state side by side, for parity); "uniform": one Dirichlet(1) draw
+    for the whole grid (a reservoir's fluid; the linear-solve workload, whose
+    Newton system is far better posed: with per-cell draws neighbouring cells
+    hold unrelated fluids and no preconditioner tried converges)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     nb = nx * ny * nz
     kx = np.exp(rng.normal(np.log(100.0), 2.0, nb))
@@ -289,9 +294,15 @@ def heterogeneous_grid(nx=100, ny=100, nz=50, Nc=6, seed=3, lbc_exact=0):
     fl.lbc_exact = lbc_exact
     rock = make_rock()
     P = 200.0 + 0.0980665 * (grid.depth - grid.depth.min()) + rng.normal(0, 10.0, nb)
-    z = rng.dirichlet(np.ones(Nc), nb)
+    if composition == "per-cell":
+        z = rng.dirichlet(np.ones(Nc), nb)
+    elif composition == "uniform":
+        z = np.tile(rng.dirichlet(np.ones(Nc)), (nb, 1))
+    else:
+        raise ValueError("composition is 'per-cell' or 'uniform'")
     st = initial_state(fl, rock, grid, P, z, rng)
-    return Workload(f"{nx}x{ny}x{nz} heterogeneous grid, Nc={Nc}", fl, make_sat(), rock, grid, st)
+    tag = "" if composition == "per-cell" else ", one composition"
+    return Workload(f"{nx}x{ny}x{nz} heterogeneous grid, Nc={Nc}{tag}", fl, make_sat(), rock, grid, st)
 
 
 def channel_facies(nx, ny, nz, rng, nchan=6, width=0.035):

@@ -102,10 +102,11 @@ def test_flash_config1(oracle, oracle_fma, mode, lanes):
 
 
 # team sizes built per Nc: 4 always, 8 from Nc = 5, 16 from Nc = 9; the
-# batched mode with (1) and without (0) the stability stages
+# batched mode with the stability and split stages (2), the stability stages
+# alone (1) and none (0)
 PE_CASES = [(Nc, m, lanes, stages) for Nc in (3, 4, 5, 9, 12)
-            for m, lanes, stages in ((PLAIN, 1, 0), (BATCHED, 1, 1), (BATCHED, 4, 1), (BATCHED, 8, 1),
-                                     (BATCHED, 16, 1), (BATCHED, 1, 0), (BATCHED, 4, 0))
+            for m, lanes, stages in ((PLAIN, 1, 0), (BATCHED, 1, 2), (BATCHED, 4, 2), (BATCHED, 8, 2),
+                                     (BATCHED, 16, 2), (BATCHED, 4, 1), (BATCHED, 1, 0), (BATCHED, 4, 0))
             if lanes < 8 or (lanes == 8 and Nc >= 5) or (lanes == 16 and Nc >= 9)]
 
 
@@ -119,7 +120,7 @@ def test_phase_equilibrium_cells(oracle, oracle_fma, Nc, mode, lanes, stages):
     wl, z = W.flash_cells(3000, Nc, seed=44 if Nc == 3 else 10 + Nc, P=np.linspace(5.0, 250.0, 3000))
     dev = IsoTFIM.from_workload(wl)
     dev.set_flash_mapping(lanes)
-    dev.set_flash_stages(bool(stages))
+    dev.set_flash_stages(stages)
     dev.PhaseEquilibrium(mode)
     NP, nu, x = dev.pe_result()
     P = wl.state["P"]

@@ -1,6 +1,9 @@
-"""Device linear solve (ocpx_linear_solve: BiCGStab + block Jacobi on the
-assembled BSR) against a direct sparse solve of the same system, and a
-Newton iteration that stays on the device (assemble -> solve -> GetSolution)."""
+"""Device linear solve (ocpx_linear_solve: ABF decoupling + FGMRES(30) with the
+CPR-type preconditioner by default, BiCGStab + block Jacobi as the other
+method) against a direct sparse solve of the same system, the reference's
+solver settings on the heterogeneous grid with and without its five wells,
+and a Newton iteration that stays on the device (assemble -> solve ->
+GetSolution)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -19,20 +22,23 @@ def _system(dev):
     return J, rhs
 
 
-def test_linear_solve_matches_direct_solve():
+@pytest.mark.parametrize("method", [1, 0])
+def test_linear_solve_matches_direct_solve(method):
     wl = W.cube_grid()
     dev = IsoTFIM.from_workload(wl)
+    dev.set_linear_solver(method)
     dev.newton_pass(1.0, BATCHED)
     J, rhs = _system(dev)
     u, it, rr = dev.LinearSolve(rtol=1e-12, maxit=2000)
-    print(f"10x10x10: {it} iterations, ||r||/||b|| {rr:.2e}")
+    print(f"10x10x10, method {method}: {it} iterations, ||r||/||b|| {rr:.2e}")
     assert it > 0 and rr < 1e-12
     true = np.linalg.norm(rhs - J @ u) / np.linalg.norm(rhs)
     ref = spla.spsolve(J.tocsc(), rhs)
     err = np.linalg.norm(u - ref) / np.linalg.norm(ref)
     print(f"10x10x10: true residual {true:.2e}, |u - u_direct| / |u_direct| {err:.2e}")
     assert true < 1e-11
-    assert err < 1e-8
+    # forward error within cond(J) times the residual: cond(J) is about 2e4 here
+    assert err < 1e-7
     dev.close()
 
 
@@ -47,6 +53,33 @@ def test_linear_solve_reference_tolerance_on_heterogeneous_grid():
     print(f"20x20x10: {it} iterations, recursive {rr:.2e}, true {true:.2e}")
     assert it > 0 and rr < 1e-3
     assert true < 2e-3
+    dev.close()
+
+
+def test_cpr_on_the_heterogeneous_grid_with_wells():
+    """The heterogeneous grid (lognormal permeability, sigma = 2, kv/kh =
+    0.1) with one composition everywhere: at 20x20x10 with its five vertical
+    wells in the system the reference's settings converge, and so they do on
+    the full 100x100x50 grid.  (Per-cell random compositions put unrelated
+    fluids side by side; their Newton system defeats every preconditioner
+    tried, an exact pressure solve included, from 50x50x25 on.)"""
+    from opencaeporox_amd import wells as WL
+    wl = W.heterogeneous_grid(20, 20, 10, Nc=6, composition="uniform")
+    dev = IsoTFIM.from_workload(wl)
+    dev.SetWells(WL.corner_and_centre_wells(wl))
+    dev.newton_pass(1.0, BATCHED)
+    dev.CalResWells(1.0, jacobian=True)
+    u, it, rr = dev.LinearSolve(rtol=1e-3, maxit=200)
+    print(f"20x20x10 with wells: {it} iterations, decoupled ||r||/||b|| {rr:.2e}")
+    assert it > 0 and rr < 1e-3
+    assert u.size == (dev.nb + 5) * dev.bs and np.all(np.isfinite(u))
+    dev.close()
+    wl = W.heterogeneous_grid(100, 100, 50, Nc=6, composition="uniform")
+    dev = IsoTFIM.from_workload(wl)
+    dev.newton_pass(1.0, BATCHED)
+    u, it, rr = dev.LinearSolve(rtol=1e-3, maxit=200, fetch=False)
+    print(f"100x100x50: {it} iterations, decoupled ||r||/||b|| {rr:.2e}")
+    assert it > 0 and rr < 1e-3
     dev.close()
 
 
