@@ -438,6 +438,23 @@ int  ocpx_set_geometry(ocpx_ctx* ctx, const double* v, const double* trans);
  * mode changes). */
 int  ocpx_newton_pass(ocpx_ctx* ctx, double dt, int mode, int use_graph);
 
+/* A batch of contexts -- for example one ocpx_ctx_create_multi context per
+ * component count -- whose Newton passes run as one graph launch.  No
+ * counterpart in the reference (one simulator instance per model).
+ * ocpx_batch_newton_pass(batch, dt) is ocpx_newton_pass(ctx, dt,
+ * OCPX_FLASH_BATCHED, 1) for every context at once: each context's pass is
+ * a branch of one captured hipGraph (forked from and joined to the batch's
+ * stream, so the contexts' kernels may overlap), launched once; it is
+ * ordered after the work already queued on each context's stream, and each
+ * context's later work after it.  Captured at the first call, again when dt
+ * or a context's flash settings change.  Contexts stay owned by the caller
+ * and must outlive the batch; hip_stream NULL: the batch makes its own.
+ * Errors: ocpx_last_error(NULL). */
+typedef struct ocpx_batch ocpx_batch;
+int  ocpx_batch_create(ocpx_ctx* const* ctxs, int32_t n, void* hip_stream, ocpx_batch** out);
+int  ocpx_batch_newton_pass(ocpx_batch* batch, double dt);
+void ocpx_batch_destroy(ocpx_batch* batch);
+
 /* Differentiation of the assembled system.
  * ocpx_jt_lambda: out = J^T lambda, J the BSR assembled last
  * (lambda, out: host [nb*(nc+1)]).

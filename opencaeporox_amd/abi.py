@@ -134,6 +134,9 @@ def load(path=None):
         "ocpx_get_flash_queue_order": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_set_flash_stages": (C.c_int, [vp, C.c_int]),
         "ocpx_set_linear_solver": (C.c_int, [vp, C.c_int]),
+        "ocpx_batch_create": (C.c_int, [C.POINTER(vp), i32, vp, C.POINTER(vp)]),
+        "ocpx_batch_newton_pass": (C.c_int, [vp, dbl]),
+        "ocpx_batch_destroy": (None, [vp]),
         "ocpx_get_linear_solver": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_phase_equilibrium_noalloc": (C.c_int, [C.POINTER(Fluid), i64, vp, vp, vp, vp, vp, vp, vp, vp]),
         "ocpx_get_flash_stages": (C.c_int, [vp, C.POINTER(i32)]),

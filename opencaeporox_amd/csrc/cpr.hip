@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "amg_layout.hpp"
 #include "cpr.hpp"
 #include "ocpx_dev.hpp"
 
@@ -17,7 +18,6 @@ namespace ocpx {
 namespace {
 
 constexpr double OMEGA = 2.0 / 3.0;          // weighted-Jacobi smoother
-constexpr double STRONG = 0.0;               // pairing candidates: w_ij >= STRONG max_k w_ik
 constexpr int MAX_LEVELS = 25;
 
 // ------------------------------------------------------------------ kernels
@@ -299,82 +299,6 @@ __global__ void __launch_bounds__(256) scatter_p(int B, const double* __restrict
 
 unsigned grid256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-// --------------------------------------------------------------- host setup
-struct Graph {
-    int64_t n = 0;
-    std::vector<int64_t> rp;
-    std::vector<int32_t> ci;
-    std::vector<double> w;
-};
-
-// Pairwise matching in row order: an unmatched row pairs with its unmatched
-// neighbour of largest weight among those within STRONG of the row's
-// largest; rows without one stay alone.  Returns the number of pairs/singles.
-int64_t pair_rows(const Graph& g, std::vector<int32_t>& map)
-{
-    map.assign(g.n, -1);
-    int64_t m = 0;
-    for (int64_t i = 0; i < g.n; i++) {
-        if (map[i] >= 0) continue;
-        double wmax = 0;
-        for (int64_t s = g.rp[i]; s < g.rp[i + 1]; s++)
-            if (g.ci[s] != i) wmax = std::max(wmax, g.w[s]);
-        int64_t best = -1;
-        double bw = 0;
-        for (int64_t s = g.rp[i]; s < g.rp[i + 1]; s++) {
-            const int64_t j = g.ci[s];
-            if (j == i || map[j] >= 0) continue;
-            if (g.w[s] >= STRONG * wmax && g.w[s] > bw) { best = j; bw = g.w[s]; }
-        }
-        map[i] = (int32_t)m;
-        if (best >= 0) map[best] = (int32_t)m;
-        m++;
-    }
-    return m;
-}
-
-// Galerkin graph of piecewise-constant aggregation: coarse pattern (diagonal
-// first), summed off-diagonal weights, and for every fine slot its coarse slot.
-void coarse_graph(const Graph& f, const std::vector<int32_t>& map, int64_t m, Graph& c,
-                  std::vector<int64_t>& slot_of, std::vector<int64_t>& mp, std::vector<int32_t>& mi)
-{
-    mp.assign(m + 1, 0);
-    for (int64_t i = 0; i < f.n; i++) mp[map[i] + 1]++;
-    for (int64_t I = 0; I < m; I++) mp[I + 1] += mp[I];
-    mi.assign(f.n, 0);
-    {
-        std::vector<int64_t> pos(mp.begin(), mp.end() - 1);
-        for (int64_t i = 0; i < f.n; i++) mi[pos[map[i]]++] = (int32_t)i;
-    }
-    c.n = m;
-    c.rp.assign(1, 0);
-    c.ci.clear();
-    c.w.clear();
-    slot_of.assign(f.ci.size(), -1);
-    std::vector<int64_t> marker(m, -1);
-    for (int64_t I = 0; I < m; I++) {
-        const int64_t start = (int64_t)c.ci.size();
-        marker[I] = start;
-        c.ci.push_back((int32_t)I);
-        c.w.push_back(0);
-        for (int64_t k = mp[I]; k < mp[I + 1]; k++) {
-            const int64_t i = mi[k];
-            for (int64_t s = f.rp[i]; s < f.rp[i + 1]; s++) {
-                const int64_t J = map[f.ci[s]];
-                if (marker[J] < start) {
-                    marker[J] = (int64_t)c.ci.size();
-                    c.ci.push_back((int32_t)J);
-                    c.w.push_back(0);
-                }
-                const int64_t q = marker[J];
-                if (J != I) c.w[q] += f.w[s];
-                slot_of[s] = q;
-            }
-        }
-        c.rp.push_back((int64_t)c.ci.size());
-    }
-}
-
 }  // namespace
 
 // ------------------------------------------------------------- PressureAmg
@@ -401,7 +325,7 @@ PressureAmg::~PressureAmg()
 int PressureAmg::setup(int64_t n, const std::vector<int64_t>& rp, const std::vector<int32_t>& col,
                        const std::vector<double>& w, std::string& err)
 {
-    Graph g;
+    amg::Graph g;
     g.n = n;
     g.rp = rp;
     g.ci = col;
@@ -417,37 +341,15 @@ int PressureAmg::setup(int64_t n, const std::vector<int64_t>& rp, const std::vec
             dev(&L.x, (size_t)L.n, (const double*)nullptr, err) ||
             dev(&L.r, (size_t)L.n, (const double*)nullptr, err))
             return -1;
-        if (g.n <= DENSE_MAX || (int)lv_.size() + 1 >= MAX_LEVELS) {
+        amg::Graph gc;
+        amg::LevelMaps M;
+        if (g.n <= DENSE_MAX || (int)lv_.size() + 1 >= MAX_LEVELS || !amg::coarsen(g, gc, M)) {
             lv_.push_back(L);
             break;
         }
-        // two rounds of pairing: aggregates of up to four rows
-        std::vector<int32_t> m1, m2;
-        const int64_t n1 = pair_rows(g, m1);
-        Graph g1;
-        std::vector<int64_t> so, mp, cp;
-        std::vector<int32_t> mi;
-        coarse_graph(g, m1, n1, g1, so, mp, mi);
-        const int64_t n2 = pair_rows(g1, m2);
-        std::vector<int32_t> agg(g.n);
-        for (int64_t i = 0; i < g.n; i++) agg[i] = m2[m1[i]];
-        if (n2 > (g.n * 9) / 10) {           // no real coarsening left: stop here
-            lv_.push_back(L);
-            break;
-        }
-        Graph gc;
-        coarse_graph(g, agg, n2, gc, so, mp, mi);
-        // gather list: coarse slot -> the fine slots summed into it
-        std::vector<int64_t> gp(gc.ci.size() + 1, 0), gs(g.ci.size());
-        for (size_t s = 0; s < so.size(); s++) gp[so[s] + 1]++;
-        for (size_t q = 0; q + 1 < gp.size(); q++) gp[q + 1] += gp[q];
-        {
-            std::vector<int64_t> pos(gp.begin(), gp.end() - 1);
-            for (size_t s = 0; s < so.size(); s++) gs[pos[so[s]]++] = (int64_t)s;
-        }
-        if (dev(&L.agg, agg.size(), agg.data(), err) || dev(&L.gp, gp.size(), gp.data(), err) ||
-            dev(&L.gs, gs.size(), gs.data(), err) || dev(&L.mp, mp.size(), mp.data(), err) ||
-            dev(&L.mi, mi.size(), mi.data(), err))
+        if (dev(&L.agg, M.agg.size(), M.agg.data(), err) || dev(&L.gp, M.gp.size(), M.gp.data(), err) ||
+            dev(&L.gs, M.gs.size(), M.gs.data(), err) || dev(&L.mp, M.mp.size(), M.mp.data(), err) ||
+            dev(&L.mi, M.mi.size(), M.mi.data(), err))
             return -1;
         lv_.push_back(L);
         g = std::move(gc);
@@ -510,31 +412,6 @@ hipError_t PressureAmg::cycle(int l, const double* b, double* x, hipStream_t st)
 hipError_t PressureAmg::vcycle(const double* b, double* x, hipStream_t st)
 {
     return cycle(0, b, x, st);
-}
-
-// ------------------------------------------------------------- colouring
-void color_rows(int64_t n, const std::vector<int64_t>& rp, const std::vector<int32_t>& col,
-                std::vector<int32_t>& cptr, std::vector<int32_t>& rows)
-{
-    std::vector<int32_t> color(n, -1);
-    std::vector<int64_t> seen;               // colour -> last row that saw it taken
-    int ncol = 0;
-    for (int64_t i = 0; i < n; i++) {
-        for (int64_t s = rp[i]; s < rp[i + 1]; s++) {
-            const int32_t c = color[col[s]];
-            if (c >= 0) seen[c] = i;
-        }
-        int c = 0;
-        while (c < ncol && seen[c] == i) c++;
-        if (c == ncol) { seen.push_back(-1); ncol++; }
-        color[i] = c;
-    }
-    cptr.assign(ncol + 1, 0);
-    for (int64_t i = 0; i < n; i++) cptr[color[i] + 1]++;
-    for (int c = 0; c < ncol; c++) cptr[c + 1] += cptr[c];
-    rows.assign(n, 0);
-    std::vector<int32_t> pos(cptr.begin(), cptr.end() - 1);
-    for (int64_t i = 0; i < n; i++) rows[pos[color[i]]++] = (int32_t)i;
 }
 
 // --------------------------------------------------------------- launchers

@@ -8,7 +8,9 @@
 // with pairwise matching on the pressure block, one V-cycle).  Here:
 //
 //  * PressureAmg: the pressure block of the decoupled system, A_p(i, j) =
-//    row 0 of D_i^{-1} times column 0 of A_ij.  Its aggregation hierarchy is
+//    row 0 of D_i^{-1} times column 0 of A_ij, made an M-matrix (positive
+//    off-diagonal entries moved to the diagonal, the diagonal at least the
+//    off-diagonal magnitudes' sum).  Its aggregation hierarchy is
 //    built once per context on the host from the transmissibility graph
 //    (pairwise matching on the strongest connection, twice per level, so
 //    aggregates of up to four cells); every solve forms the level matrices
@@ -73,11 +75,6 @@ private:
     int dev(T** p, size_t count, const T* host, std::string& err);
     hipError_t cycle(int l, const double* b, double* x, hipStream_t st);
 };
-
-// Greedy colouring of the block graph (rowptr, col): colour lists
-// rows[cptr[c] .. cptr[c+1]) in ascending row order.
-void color_rows(int64_t n, const std::vector<int64_t>& rp, const std::vector<int32_t>& col,
-                std::vector<int32_t>& cptr, std::vector<int32_t>& rows);
 
 // x_i = rhat_i - Dinv_i sum_{j != i} A_ij x_j for the rows of one colour
 hipError_t run_gs_color(int B, const int32_t* rows, int64_t nrows, const int64_t* rowptr, const int32_t* col,

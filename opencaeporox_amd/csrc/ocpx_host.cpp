@@ -18,6 +18,7 @@
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 #include "layout.hpp"
+#include "amg_layout.hpp"
 #include "cpr.hpp"
 
 namespace ocpx {
@@ -129,6 +130,7 @@ struct ocpx_ctx {
     hipGraphExec_t gexec = nullptr;
     double gdt = 0;
     int gmode = -1;
+    uint64_t gen = 0;            // settings generation: changes whenever a captured pass goes stale
     std::vector<void*> allocs;
 };
 
@@ -408,6 +410,7 @@ int launch_res_asm(ocpx_ctx* c, double dt, bool dr, bool dj)
 
 int destroy_graph(ocpx_ctx* c)
 {
+    c->gen++;                    // graphs of batches holding this context are stale too
     if (c->gexec) hipGraphExecDestroy(c->gexec);
     if (c->graph) hipGraphDestroy(c->graph);
     c->gexec = nullptr;
@@ -1277,7 +1280,7 @@ int cpr_setup(ocpx_ctx* c)
     c->amg = std::move(amg);
     if (!c->gs_rows) {
         std::vector<int32_t> rows;
-        color_rows(nb, rp, col, c->gs_cptr, rows);
+        amg::color_rows(nb, rp, col, c->gs_cptr, rows);
         int rc = dalloc(c, &c->gs_rows, rows.size());
         if (rc) return rc;
         HIPCHK(c, hipMemcpy(c->gs_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
@@ -1669,6 +1672,166 @@ int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
     c->wells_applied = false;
     c->res_current = true;
     return OCPX_OK;
+}
+
+}  // extern "C"
+
+// Several contexts whose Newton passes are captured into one graph: each
+// context's flash + kr/Pc + rock + residual + assembly is a branch forked
+// from and joined to the batch stream, so one graph launch runs every
+// context's pass, the branches free to overlap on the device.
+struct ocpx_batch {
+    std::vector<ocpx_ctx*> ctx;
+    std::vector<uint64_t> gen;   // the contexts' settings generations at capture
+    hipStream_t st = nullptr;
+    bool own_stream = false;
+    hipEvent_t fork = nullptr, done = nullptr;
+    std::vector<hipEvent_t> ev;  // per context: its join into the capture, its work before a launch
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double gdt = 0;
+};
+
+namespace {
+
+void batch_drop_graph(ocpx_batch* b)
+{
+    if (b->gexec) hipGraphExecDestroy(b->gexec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+int batch_fail(int code, const std::string& msg)
+{
+    set_call_error("ocpx_batch: " + msg);
+    return code;
+}
+
+#define BCHK(call)                                                                                 \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return batch_fail(OCPX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int batch_capture(ocpx_batch* b, double dt)
+{
+    batch_drop_graph(b);
+    for (ocpx_ctx* c : b->ctx)
+        if (c->stages && !c->mid) {                  // no allocation inside the capture
+            int rc = sta_alloc(c);
+            if (rc) return batch_fail(rc, c->err);
+        }
+    std::vector<bool> tm(b->ctx.size());
+    for (size_t k = 0; k < b->ctx.size(); k++) { tm[k] = b->ctx[k]->timing; b->ctx[k]->timing = false; }
+    BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipEventRecord(b->fork, b->st);
+    int rc = e == hipSuccess ? OCPX_OK : OCPX_E_HIP;
+    std::string why = e == hipSuccess ? "" : std::string("fork: ") + hipGetErrorString(e);
+    for (size_t k = 0; k < b->ctx.size() && !rc; k++) {
+        ocpx_ctx* c = b->ctx[k];
+        if ((e = hipStreamWaitEvent(c->st, b->fork, 0)) != hipSuccess) {
+            rc = OCPX_E_HIP;
+            why = std::string("branch: ") + hipGetErrorString(e);
+            break;
+        }
+        rc = launch_pe(c, OCPX_FLASH_BATCHED);
+        if (!rc) rc = launch_tail(c);
+        if (!rc) rc = launch_krpc_rock(c);
+        if (!rc) rc = launch_res_asm(c, dt, true, true);
+        if (rc) { why = c->err; break; }
+        if ((e = hipEventRecord(b->ev[k], c->st)) != hipSuccess || (e = hipStreamWaitEvent(b->st, b->ev[k], 0)) != hipSuccess) {
+            rc = OCPX_E_HIP;
+            why = std::string("join: ") + hipGetErrorString(e);
+        }
+    }
+    hipGraph_t gph = nullptr;
+    e = hipStreamEndCapture(b->st, &gph);
+    for (size_t k = 0; k < b->ctx.size(); k++) b->ctx[k]->timing = tm[k];
+    if (rc) { if (gph) hipGraphDestroy(gph); return batch_fail(rc, why); }
+    if (e != hipSuccess) return batch_fail(OCPX_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+    b->graph = gph;
+    BCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    b->gdt = dt;
+    for (size_t k = 0; k < b->ctx.size(); k++) b->gen[k] = b->ctx[k]->gen;
+    return OCPX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ocpx_batch_create(ocpx_ctx* const* ctxs, int32_t n, void* hip_stream, ocpx_batch** out)
+{
+    clear_call_error();
+    if (!ctxs || n <= 0 || !out) return batch_fail(OCPX_E_ARG, "needs at least one context and an output");
+    for (int32_t k = 0; k < n; k++) {
+        if (!ctxs[k]) return batch_fail(OCPX_E_ARG, "null context");
+        for (int32_t q = 0; q < k; q++)
+            if (ctxs[q] == ctxs[k]) return batch_fail(OCPX_E_ARG, "a context appears twice");
+        if (ctxs[k]->mid && ctxs[k]->pe_lanes == 1)
+            return batch_fail(OCPX_E_UNSUPPORTED, "contexts of several models flash with a team mapping");
+    }
+    auto* b = new ocpx_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->gen.assign(n, ~0ull);
+    b->ev.assign(n, nullptr);
+    b->st = (hipStream_t)hip_stream;
+    bool ok = true;
+    if (!b->st) {
+        ok = hipStreamCreate(&b->st) == hipSuccess;
+        b->own_stream = ok;
+    }
+    ok = ok && hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    for (int32_t k = 0; k < n && ok; k++) ok = hipEventCreateWithFlags(&b->ev[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        ocpx_batch_destroy(b);
+        return batch_fail(OCPX_E_HIP, "stream or event creation failed");
+    }
+    *out = b;
+    return OCPX_OK;
+}
+
+int ocpx_batch_newton_pass(ocpx_batch* b, double dt)
+{
+    clear_call_error();
+    if (!b) return batch_fail(OCPX_E_ARG, "null batch");
+    bool stale = !b->gexec || b->gdt != dt;
+    for (size_t k = 0; k < b->ctx.size(); k++) stale = stale || b->gen[k] != b->ctx[k]->gen;
+    if (stale) {
+        int rc = batch_capture(b, dt);
+        if (rc) return rc;
+    }
+    // the launch follows what was queued on each context's stream, and each
+    // context's later work follows the launch
+    for (size_t k = 0; k < b->ctx.size(); k++) {
+        BCHK(hipEventRecord(b->ev[k], b->ctx[k]->st));
+        BCHK(hipStreamWaitEvent(b->st, b->ev[k], 0));
+    }
+    BCHK(hipGraphLaunch(b->gexec, b->st));
+    BCHK(hipEventRecord(b->done, b->st));
+    for (ocpx_ctx* c : b->ctx) {
+        BCHK(hipStreamWaitEvent(c->st, b->done, 0));
+        c->flashed = c->assembled = true;
+        c->wells_jac = false;
+        c->wells_applied = false;
+        c->res_current = true;
+    }
+    return OCPX_OK;
+}
+
+void ocpx_batch_destroy(ocpx_batch* b)
+{
+    if (!b) return;
+    if (b->st) hipStreamSynchronize(b->st);
+    batch_drop_graph(b);
+    if (b->fork) hipEventDestroy(b->fork);
+    if (b->done) hipEventDestroy(b->done);
+    for (hipEvent_t e : b->ev)
+        if (e) hipEventDestroy(e);
+    if (b->own_stream) hipStreamDestroy(b->st);
+    delete b;
 }
 
 int ocpx_jt_lambda_dev(ocpx_ctx* c, const double* lam, double* out)

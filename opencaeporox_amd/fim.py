@@ -172,7 +172,7 @@ class IsoTFIM:
     def flash_stages(self):
         v = C.c_int32()
         self._chk(self.lib.ocpx_get_flash_stages(self.h, C.byref(v)), "flash stages")
-        return bool(v.value)
+        return v.value
 
     def CalFlash(self, mode=PLAIN):
         self._chk(self.lib.ocpx_flash_der(self.h, mode), "flash")
@@ -329,7 +329,8 @@ class IsoTFIM:
 class MultiModelBatch:
     """Independent models of uneven size and component count: one context per
     Nc holding all models of that Nc (ocpx_ctx_create_multi), each context
-    with its own autotuned flash mapping and its own captured Newton step."""
+    with its own autotuned flash mapping; a Newton step of every model is one
+    graph launch (ocpx_batch_newton_pass)."""
 
     def __init__(self, wls, stream=None):
         groups = {}
@@ -337,6 +338,8 @@ class MultiModelBatch:
             groups.setdefault(w.fluid.nc, []).append(m)
         self.models = wls
         self.groups = {}          # Nc -> (context, model indices)
+        self._batch = None
+        self.lib = abi.load()
         for Nc in sorted(groups):
             idx = groups[Nc]
             self.groups[Nc] = (IsoTFIM.from_workloads([wls[m] for m in idx], stream), idx)
@@ -364,8 +367,23 @@ class MultiModelBatch:
             d.ResetToLastTimeStep()
 
     def newton_pass(self, dt, graph=False):
-        for d in self.contexts():
-            d.newton_pass(dt, BATCHED, graph=graph)
+        """graph=True: every context's pass in one captured graph, one launch
+        (ocpx_batch_newton_pass); False: the contexts' passes one after another."""
+        if not graph:
+            for d in self.contexts():
+                d.newton_pass(dt, BATCHED, graph=False)
+            return
+        if self._batch is None:
+            ctxs = self.contexts()
+            arr = (C.c_void_p * len(ctxs))(*[d.h.value for d in ctxs])
+            out = C.c_void_p()
+            rc = self.lib.ocpx_batch_create(arr, len(ctxs), None, C.byref(out))
+            if rc != 0:
+                raise OcpxError(f"ocpx_batch_create failed ({rc}): {self.lib.ocpx_last_error(None).decode()}")
+            self._batch = out
+        rc = self.lib.ocpx_batch_newton_pass(self._batch, dt)
+        if rc != 0:
+            raise OcpxError(f"ocpx_batch_newton_pass failed ({rc}): {self.lib.ocpx_last_error(None).decode()}")
 
     def sync(self):
         for d in self.contexts():
@@ -381,5 +399,8 @@ class MultiModelBatch:
         return a[d.offsets[k] * L:d.offsets[k + 1] * L]
 
     def close(self):
+        if self._batch is not None:
+            self.lib.ocpx_batch_destroy(self._batch)
+            self._batch = None
         for d in self.contexts():
             d.close()

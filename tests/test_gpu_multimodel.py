@@ -72,3 +72,29 @@ def test_multimodel_batch_groups_by_component_count():
             assert np.allclose(batch.get(m, f), one.get(f), rtol=1e-12, atol=0), f"model {m} {f}"
         one.close()
     batch.close()
+
+
+def test_batch_graph_replays_equal_the_contexts_passes():
+    """One graph launch per Newton step for every Nc group (ocpx_batch_newton_pass),
+    replayed three times from the last time step with eager work in between:
+    each replay equals the groups' own passes run one after another."""
+    wls = W.uneven_batch(nmodels=8, seed=5, cmin=300, cmax=3000)
+    a, b = MultiModelBatch(wls), MultiModelBatch(wls)
+    for d, _ in a.groups.values():
+        d.set_flash_mapping(4)
+    for d, _ in b.groups.values():
+        d.set_flash_mapping(4)
+    a.UpdateLastTimeStep()
+    b.UpdateLastTimeStep()
+    b.newton_pass(1.0, graph=False)
+    b.sync()
+    ref = {Nc: [d.get(f) for f in ("S", "DSEC_DPRI")] + list(d.bsr()[2:]) for Nc, (d, _) in b.groups.items()}
+    for rep in range(3):
+        a.ResetToLastTimeStep()
+        a.newton_pass(1.0, graph=True)
+        for Nc, (d, _) in a.groups.items():
+            got = [d.get(f) for f in ("S", "DSEC_DPRI")] + list(d.bsr()[2:])
+            for x, y in zip(got, ref[Nc]):
+                assert np.array_equal(x, y), f"replay {rep}, Nc = {Nc}"
+    a.close()
+    b.close()
