@@ -96,6 +96,19 @@ OCPX_HD const DevFluid* opaque(const DevFluid* p)
     return p;
 }
 
+// The fluid constants seen through the constant address space: the lean
+// kernels' uniform reads of them become scalar loads into SGPRs instead of
+// vector loads into VGPRs.  The asm makes the pointer opaque where it is
+// taken, so each use reloads what it needs rather than holding the
+// binary-interaction matrix in registers across the loop.
+using CFluid = __attribute__((address_space(4))) const DevFluid;
+__device__ __forceinline__ CFluid* cfl(const DevFluid* p)
+{
+    CFluid* q = (CFluid*)p;
+    asm volatile("" : "+s"(q));
+    return q;
+}
+
 // Append to a device list with one atomic per wavefront.
 __device__ __forceinline__ void append(bool want, unsigned* count, int32_t* list, int32_t value)
 {
@@ -114,8 +127,8 @@ enum : int { S_IDLE = 0, S_FEED, S_START, S_SSM };
 // ln phi at y (src/OCPEoS.cpp:68-124, 199-214 in log form) with the cell's
 // sqrt(Ai) in registers and Bi = bT P / T formed where used, so a lane holds
 // only what the SSM loop carries
-template <int NC>
-DEV void lean_lnphi(const DevFluid& f, const double* sA, double PT, const double* y, double* ev)
+template <int NC, class F>
+DEV void lean_lnphi(const F& f, const double* sA, double PT, const double* y, double* ev)
 {
     double w[NC];
     UNROLL for (int k = 0; k < NC; k++) w[k] = sA[k] * y[k];
@@ -197,7 +210,7 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
         if (!__any(st != S_IDLE)) break;
         if (st == S_IDLE) continue;
 
-        const DevFluid& f = *opaque(fp);
+        CFluid& f = *cfl(fp);
         // the composition evaluated in this step (the feed's is in y already)
         if (st == S_START) {       // Y = x / Kw[k] (:420-425), x re-read from Ni
             const double ck = trial_coef(k);
@@ -286,6 +299,9 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 // stage 2: StableNR (:475-521) of a queued trial, one lane each, from its
 // SSM's last Y and Yt.  Jacobian JmatSTA (:523-541) solved as the symmetric
 // matrix of its lower triangle (dsysv_ 'U' on row-major storage, :493).
+// Every Jacobian row and every evaluation reads the fluid constants through
+// its own opaque pointer, so the binary-interaction matrix is not held in
+// registers across the solve.
 template <int NC>
 __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
@@ -321,7 +337,7 @@ __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp,
             double S[NC][NC], rhs[1][NC];
             UNROLL for (int i = 0; i < NC; i++) {
                 double row[NC];
-                lnfugx_row<NC>(f, ec, mx, Zx, y, P, i, row);
+                lnfugx_row<NC>(*opaque(fp), ec, mx, Zx, y, P, i, row);
                 double t = 0;
                 UNROLL for (int j = 0; j < NC; j++) t += y[j] * row[j];
                 UNROLL for (int j = 0; j <= i; j++) {
@@ -338,7 +354,7 @@ __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp,
             Yt = yt;
             const double r = 1 / yt;
             UNROLL for (int i = 0; i < NC; i++) y[i] *= r;
-            eos_mix<NC>(f, ec, y, mx);
+            eos_mix<NC>(*opaque(fp), ec, y, mx);
             eos_lnphi<NC>(ec, mx, ev);
             evals++;
             e2 = 0;
@@ -498,11 +514,14 @@ pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
             }
             rr += it;
         }
+        // each evaluation reads the fluid constants through its own opaque
+        // pointer, so the binary-interaction matrix is not held in registers
+        // from one evaluation to the next
         double x[NC], l0[NC], l1[NC];
         UNROLL for (int i = 0; i < NC; i++) x[i] = K[i] * (z[i] / (1 + nu * (K[i] - 1)));     // x0
-        lean_lnphi<NC>(f, sA, PT, x, l0);
+        lean_lnphi<NC>(*cfl(fp), sA, PT, x, l0);
         UNROLL for (int i = 0; i < NC; i++) x[i] = z[i] / (1 + nu * (K[i] - 1));              // x1
-        lean_lnphi<NC>(f, sA, PT, x, l1);
+        lean_lnphi<NC>(*cfl(fp), sA, PT, x, l1);
         iter++;
         double Se = 0;
         UNROLL for (int i = 0; i < NC; i++) {

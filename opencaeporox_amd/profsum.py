@@ -1,6 +1,8 @@
-"""Per-kernel summary of a rocprofv3 kernel-trace database (rocpd SQLite).
+"""Per-kernel summary of a rocprofv3 kernel trace: the rocpd SQLite
+database (RUN_results.db) or the CSV trace (RUN_kernel_trace.csv, from
+--output-format csv).
 
-    python -m opencaeporox_amd.profsum RUN_results.db [--last N] [--kernel SUBSTR]
+    python -m opencaeporox_amd.profsum RUN_results.db|RUN_kernel_trace.csv [--last N] [--kernel SUBSTR]
 
 Prints, per kernel name: calls, total / average / min / max duration (ns)
 and the register and scratch footprint of the code object.  --last N keeps
@@ -8,11 +10,20 @@ only the last N dispatches of each kernel (e.g. the timed steps of a bench
 run that also launched the kernel while preparing its input).
 """
 import argparse
+import csv
 import sqlite3
 import sys
 
 
 def kernel_rows(db):
+    if db.endswith(".csv"):
+        with open(db) as f:
+            rows = [(r["Kernel_Name"], int(r["Start_Timestamp"]), int(r["End_Timestamp"]), int(r["VGPR_Count"]),
+                     int(r["Accum_VGPR_Count"]), int(r["SGPR_Count"]), int(r["Scratch_Size"]),
+                     int(r["LDS_Block_Size"]), int(r["Grid_Size_X"]), int(r["Workgroup_Size_X"]))
+                    for r in csv.DictReader(f)]
+        rows.sort(key=lambda r: r[1])
+        return rows
     con = sqlite3.connect(db)
     q = ("select name, start, end, vgpr_count, accum_vgpr_count, sgpr_count, scratch_size, "
          "lds_size, grid_x, workgroup_x from kernels order by start")
