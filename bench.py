@@ -461,7 +461,8 @@ def main():
         pe_kernel = f"pe_batched<{a.Nc}>"
     # the batched flash is one launch group: the stages, then the mapped kernel
     pe_group = ([f"pe_sta_ssm<{a.Nc}>", f"pe_sta_nr<{a.Nc}>", f"pe_sta_join<{a.Nc}>"]
-                + ([f"pe_sta_split<{a.Nc}>"] if stages >= 2 else []) if stages else []) + [pe_kernel]
+                + ([f"pe_sta_split<{a.Nc}>"] if stages >= 2 else [])
+                + ([f"pe_sta_snr<{a.Nc}>"] if stages >= 3 else []) if stages else []) + [pe_kernel]
     group_traffic = [measured_traffic(k, traffic_key(a)) for k in pe_group]
     pe_traffic = sum(group_traffic) if all(v is not None for v in group_traffic) else None
     tflops = head["flops"] / (head["kernel_ms"]["pe"] * 1e-3) / 1e12
@@ -476,6 +477,7 @@ def main():
                    "flash": "batched" if "batched" in out else modes[-1],
                    "lanes_per_cell": dev.flash_mapping() if "batched" in out else 1,
                    "flash_stages": stages,
+                   "flash_long_evals": dev.flash_long() if "batched" in out else 0,
                    "queue_longest_first": dev.flash_queue_order() if "batched" in out else False,
                    "autotune_ms": tune},
         "roofline": {"bound": "mfma", "achieved": tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",

@@ -287,12 +287,29 @@ int  ocpx_autotune_flash(ocpx_ctx* ctx, double* ms);
  * of the trials that need one in a second, then per cell the first unstable
  * trial in the reference's order -- and the mapped kernel of
  * ocpx_set_flash_mapping takes only the cells left (splits, flash types 1
- * and 2, the relaxed test); 2 (default): as 1, and the first SplitSSM2 pass
- * of the unstable cells (:586-649) runs in a lean kernel of its own, the
- * mapped kernel resuming them at SplitNR (:807-823); 0: the mapped kernel
- * takes every cell.  Contexts of several models always run unstaged. */
+ * and 2, the relaxed test); 2: as 1, and the first SplitSSM2 pass of the
+ * unstable cells (:586-649) runs in a lean kernel of its own, the mapped
+ * kernel resuming them at SplitNR (:807-823); 3 (default): as 2, and SplitNR
+ * with CheckSplit (:805-1010) in a lean kernel too, the mapped kernel taking
+ * only the cells whose Newton steps do not converge, at the restart loop
+ * (:558-566); 0: the mapped kernel takes every cell.  Contexts of several
+ * models always run unstaged. */
 int  ocpx_set_flash_stages(ocpx_ctx* ctx, int on);
 int  ocpx_get_flash_stages(ocpx_ctx* ctx, int32_t* on);
+/* Long cells of the staged batched flash.  No counterpart in the reference;
+ * the same algorithm either way.  Cells whose previous batched flash took more
+ * than `evals` fugacity evaluations (ITERS row 5; default 1000, 0 = off) are
+ * flashed from the start by the mapped kernel on a branch of their own beside
+ * the stages (a second stream forked from and joined to the context's,
+ * inside the context's own captured graph too; off inside a batch graph,
+ * ocpx_batch_newton_pass): their serial Newton and restart work, not the
+ * bulk, is the flash's critical path.  Such a cell runs the mapped kernel's
+ * arithmetic instead of the stages', so its last bits follow the counts in
+ * OCPX_F_ITERS when the flash starts; those are saved and restored with the
+ * last time step, so a pass repeated after ocpx_reset_to_last_step repeats
+ * its bits. */
+int  ocpx_set_flash_long(ocpx_ctx* ctx, int32_t evals);
+int  ocpx_get_flash_long(ocpx_ctx* ctx, int32_t* evals);
 /* Queue order of the batched flash.  No counterpart in the reference (its
  * flash loops over cells in order); same results either way.  1 (default):
  * each batched flash hands out the cells longest first, by their fugacity-
@@ -416,8 +433,10 @@ int  ocpx_get_linear_solver(ocpx_ctx* ctx, int32_t* method);
 /* ≙ IsoT_FIM::UpdateLastTimeStep (src/IsoThermalMethod.cpp:1473-1515) with
  * SkipPSAVarset::UpdateLastTimeStep (src/AcceleratePEC.cpp:24-31): keeps a
  * device copy of the bulk state (rock, Nt, vf, P, Pj, Pc, phaseExist, S,
- * xij, rho, xi, mu, kr, their derivatives, dSec_dPri, the SkipPSA data) and
- * sets lNi (OCPX_F_LNI) = Ni.  Device-to-device, on the context stream. */
+ * xij, rho, xi, mu, kr, their derivatives, dSec_dPri, the SkipPSA data, and
+ * the flash counters OCPX_F_ITERS, which pick the long cells of the next
+ * batched flash) and sets lNi (OCPX_F_LNI) = Ni.  Device-to-device, on the
+ * context stream. */
 int  ocpx_update_last_step(ocpx_ctx* ctx);
 /* ≙ IsoT_FIM::ResetToLastTimeStep (:1420-1464) with
  * SkipPSAVarset::ResetToLastTimeStep (src/AcceleratePEC.cpp:15-22):

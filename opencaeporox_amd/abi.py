@@ -134,6 +134,8 @@ def load(path=None):
         "ocpx_get_flash_queue_order": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_set_flash_stages": (C.c_int, [vp, C.c_int]),
         "ocpx_set_linear_solver": (C.c_int, [vp, C.c_int]),
+        "ocpx_set_flash_long": (C.c_int, [vp, i32]),
+        "ocpx_get_flash_long": (C.c_int, [vp, C.POINTER(i32)]),
         "ocpx_batch_create": (C.c_int, [C.POINTER(vp), i32, vp, C.POINTER(vp)]),
         "ocpx_batch_newton_pass": (C.c_int, [vp, dbl]),
         "ocpx_batch_destroy": (None, [vp]),

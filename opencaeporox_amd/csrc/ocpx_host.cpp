@@ -91,11 +91,14 @@ struct ocpx_ctx {
     int pe_lanes_auto = 0;       // the autotuner's choice, 0 if it has not run
     int team_blocks[3] = {0, 0, 0}; // persistent grid of the team kernels, T = 4 / 8 / 16
     int ncu = 256;               // compute units of the device
-    int sta_grid[3] = {0, 0, 0}; // persistent grids of the flash stages (stability SSM, Newton, split)
-    int stages = 2;              // batched flash: 1 strict stability test in its own stages, 2 and
-                                 // the split's first SSM pass (pe_sta.hip), 0 none
+    int sta_grid[4] = {0, 0, 0, 0}; // persistent grids of the flash stages (stability SSM, Newton, split SSM, split Newton)
+    int stages = 3;              // batched flash: 1 strict stability test in its own stages, 2 and
+                                 // the split's first SSM pass, 3 and its Newton steps (pe_sta.hip), 0 none
     bool sta_ready = false;      // the stage buffers are allocated
     StaIO sta{};
+    int long_evals = 1000;       // staged flash: cells over this many evaluations last time run beside the stages
+    hipStream_t st2 = nullptr;   // the branch that runs them, forked from and joined to st by events
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // Peaceman wells (ocpx_set_wells)
     int32_t nwell = 0, nperf = 0;
     ocpx_well* wells = nullptr;  // [nwell] device copy
@@ -279,14 +282,15 @@ int sta_alloc(ocpx_ctx* c)
     StaIO& s = c->sta;
     unsigned* cnt = nullptr;
     int rc;
-    if ((rc = dalloc(c, &cnt, 8)) || (rc = dalloc(c, &s.glist, n)) || (rc = dalloc(c, &s.route, n)) ||
+    if ((rc = dalloc(c, &cnt, 16)) || (rc = dalloc(c, &s.glist, n)) || (rc = dalloc(c, &s.route, n)) ||
         (rc = dalloc(c, &s.decided, n)) || (rc = dalloc(c, &s.defer, n)) || (rc = dalloc(c, &s.Ytd, n)) ||
         (rc = dalloc(c, &s.evals, n)) || (rc = dalloc(c, &s.its, 4 * n)) || (rc = dalloc(c, &s.item, 4 * n)) ||
         (rc = dalloc(c, &s.di, (size_t)c->NC * n)) || (rc = dalloc(c, &s.nr_cell, cap)) ||
         (rc = dalloc(c, &s.nr_k, cap)) || (rc = dalloc(c, &s.nr_flag, cap)) || (rc = dalloc(c, &s.nr_its, cap)) ||
         (rc = dalloc(c, &s.nr_evals, cap)) || (rc = dalloc(c, &s.nr_Yt, cap)) ||
         (rc = dalloc(c, &s.nr_y, (size_t)c->NC * cap)) || (rc = dalloc(c, &s.rs_Yt, n)) ||
-        (rc = dalloc(c, &s.rs_cnt, 8 * n)) || (rc = dalloc(c, &s.slist, n)) ||
+        (rc = dalloc(c, &s.rs_cnt, 8 * n)) || (rc = dalloc(c, &s.slist, n)) || (rc = dalloc(c, &s.tlist, n)) ||
+        (rc = dalloc(c, &s.llist, n)) ||
         (rc = dalloc(c, &s.rs_v, (size_t)5 * c->NC * n)) || (rc = dalloc(c, &s.rs_nu, n)))
         return rc;
     s.qhead = cnt;
@@ -294,12 +298,23 @@ int sta_alloc(ocpx_ctx* c)
     s.ncount = cnt + 2;
     s.scount = cnt + 3;
     s.shead = cnt + 4;
+    s.tcount = cnt + 5;
+    s.thead = cnt + 6;
+    s.lcount = cnt + 7;
+    s.lqueue = cnt + 8;
+    if (!c->st2) {
+        if (hipStreamCreate(&c->st2) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+            return fail(c, OCPX_E_HIP, "flash branch stream or events");
+    }
     s.nr_cap = cap;
     c->sta_ready = true;
     return OCPX_OK;
 }
 
-int launch_pe(ocpx_ctx* c, int mode)
+// branch_ok: the long cells may run on the context's second stream (off when
+// the context's own stream is itself a branch of a batch capture)
+int launch_pe(ocpx_ctx* c, int mode, bool branch_ok = true)
 {
     PeIO io{};
     io.nb = c->nb;
@@ -326,10 +341,31 @@ int launch_pe(ocpx_ctx* c, int mode)
         int rc = sta_alloc(c);
         if (rc) return rc;
     }
+    bool branched = false;
     {
         TimerScope t(c, 0);
         if (staged) {
             c->sta.split = c->stages >= 2;
+            c->sta.snr = c->stages >= 3;
+            // the cells that took longest last time run beside the stages from the
+            // start (their serial Newton and restart work is the flash's critical path)
+            const bool branch = branch_ok && c->long_evals > 0 && mode == OCPX_FLASH_BATCHED;
+            c->sta.long_evals = branch ? c->long_evals : 0;
+            c->sta.prev_evals = io.iters + (size_t)5 * c->nb;
+            HIPCHK(c, run_pe_sta_prep(c->sta, c->nb, c->st));
+            if (branch) {
+                PeIO lio = io;
+                lio.qorder = c->sta.llist;
+                lio.nq = c->sta.lcount;
+                lio.queue = c->sta.lqueue;
+                HIPCHK(c, hipEventRecord(c->ev_fork, c->st));
+                HIPCHK(c, hipStreamWaitEvent(c->st2, c->ev_fork, 0));
+                const int T = c->pe_lanes;
+                if (T > 1) HIPCHK(c, run_pe_team(c->NC, T, c->df, lio, true, std::min(c->team_blocks[team_slot(T)], 512), c->st2));
+                else HIPCHK(c, run_pe(c->NC, c->df, lio, OCPX_FLASH_BATCHED, std::min(c->grid_waves, 512), c->st2));
+                HIPCHK(c, hipEventRecord(c->ev_join, c->st2));
+                branched = true;
+            }
             HIPCHK(c, run_pe_sta(c->NC, c->df, io, c->sta, c->sta_grid, c->st));
             io.qorder = c->sta.glist;
             io.nq = c->sta.gcount;
@@ -345,6 +381,7 @@ int launch_pe(ocpx_ctx* c, int mode)
         } else {
             HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
         }
+        if (branched) HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
     }
     if (mode == OCPX_FLASH_BATCHED) {
         // the next batched flash takes the cells longest first, by this one's
@@ -623,7 +660,7 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     c->ncu = ncu > 0 ? ncu : 256;
-    for (int q = 0; q < 3; q++) c->sta_grid[q] = c->ncu * pe_sta_blocks_per_cu(c->NC, q);
+    for (int q = 0; q < 4; q++) c->sta_grid[q] = c->ncu * pe_sta_blocks_per_cu(c->NC, q);
     const int per_cu = pe_batched_blocks_per_cu(c->NC);
     c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
     for (int q = 0; q < 3; q++) {
@@ -690,6 +727,9 @@ void ocpx_ctx_destroy(ocpx_ctx* c)
         for (auto& p : t.pending) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
     for (void* p : c->allocs) hipFree(p);
     if (c->own_stream) hipStreamDestroy(c->st);
+    if (c->st2) hipStreamDestroy(c->st2);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
     delete c;
 }
 
@@ -847,6 +887,21 @@ int ocpx_autotune_flash(ocpx_ctx* c, double* ms)
     return ocpx_set_flash_mapping(c, best);
 }
 
+int ocpx_set_flash_long(ocpx_ctx* c, int32_t evals)
+{
+    if (!c || evals < 0) return OCPX_E_ARG;
+    if (evals != c->long_evals) destroy_graph(c);   // (also marks batch graphs stale)
+    c->long_evals = evals;
+    return OCPX_OK;
+}
+
+int ocpx_get_flash_long(ocpx_ctx* c, int32_t* evals)
+{
+    if (!c || !evals) return OCPX_E_ARG;
+    *evals = c->long_evals;
+    return OCPX_OK;
+}
+
 int ocpx_set_linear_solver(ocpx_ctx* c, int method)
 {
     if (!c || (method != 0 && method != 1)) return OCPX_E_ARG;
@@ -863,7 +918,7 @@ int ocpx_get_linear_solver(ocpx_ctx* c, int32_t* method)
 
 int ocpx_set_flash_stages(ocpx_ctx* c, int on)
 {
-    if (!c || on < 0 || on > 2) return OCPX_E_ARG;
+    if (!c || on < 0 || on > 3) return OCPX_E_ARG;
     if (on != c->stages) destroy_graph(c);
     c->stages = on;
     return OCPX_OK;
@@ -1578,7 +1633,8 @@ const int kLastStepFields[] = {
     OCPX_F_PC, OCPX_F_PHASEEXIST, OCPX_F_S, OCPX_F_XIJ, OCPX_F_RHO, OCPX_F_XI, OCPX_F_MU,
     OCPX_F_KR, OCPX_F_VFP, OCPX_F_VFI, OCPX_F_RHOP, OCPX_F_RHOX, OCPX_F_XIP, OCPX_F_XIX,
     OCPX_F_MUP, OCPX_F_MUX, OCPX_F_DPCDS, OCPX_F_DKRDS, OCPX_F_DSEC_DPRI, OCPX_F_SKIP_FLAG,
-    OCPX_F_SKIP_MINEIG, OCPX_F_SKIP_P, OCPX_F_SKIP_T, OCPX_F_SKIP_ZI};
+    OCPX_F_SKIP_MINEIG, OCPX_F_SKIP_P, OCPX_F_SKIP_T, OCPX_F_SKIP_ZI,
+    OCPX_F_ITERS};   // the counters route the next staged flash's long cells
 
 size_t field_bytes(const ocpx_ctx* c, int fid) { return (size_t)c->nb * c->f[fid].len * c->f[fid].esz; }
 }  // namespace
@@ -1735,7 +1791,10 @@ int batch_capture(ocpx_batch* b, double dt)
             why = std::string("branch: ") + hipGetErrorString(e);
             break;
         }
-        rc = launch_pe(c, OCPX_FLASH_BATCHED);
+        // the long-cell branch stays off here: its fork from a context stream
+        // that is itself a branch of this capture is not supported (the
+        // contexts' branches of the batch overlap on the device instead)
+        rc = launch_pe(c, OCPX_FLASH_BATCHED, false);
         if (!rc) rc = launch_tail(c);
         if (!rc) rc = launch_krpc_rock(c);
         if (!rc) rc = launch_res_asm(c, dt, true, true);

@@ -42,7 +42,8 @@ struct PeIO {
     // what the split stage left (pe_sta.hip); null: no stages
     const int32_t* sta_route;   // [nb]
     const double* rs_Yt;        // [nb]
-    const int32_t* rs_cnt;      // [8][nb]: ITERS rows 0, 1, 5, 6 (= row 7), 2, 4; SSM converged
+    const int32_t* rs_cnt;      // [8][nb]: ITERS rows 0, 1, 5, 6, 2, 4; SSM converged; row 3
+                                // (row 7 = row 6 - row 3)
     const double* rs_v;         // [5][Nc][nb]: x0, x1, K, ln phi(x0), ln phi(x1)
     const double* rs_nu;        // [nb]
 };
@@ -58,12 +59,26 @@ struct StaIO {
     unsigned* gcount;           //   cells for the general kernel,
     unsigned* ncount;           //   queued Newton trials,
     unsigned* scount;           //   cells for the split stage,
-    unsigned* shead;            //   split-stage queue head
+    unsigned* shead;            //   split-stage queue head,
+    unsigned* tcount;           //   cells for the split-Newton stage,
+    unsigned* thead;            //   split-Newton queue head
     int32_t* glist;             // [nb] cells for the general kernel
     int32_t* slist;             // [nb] cells for the split stage
+    int32_t* tlist;             // [nb] cells for the split-Newton stage
     int split;                  // 1: unstable cells take the split stage before the general kernel
+    int snr;                    // 1: and then the split-Newton stage
+    // cells whose previous flash took more than long_evals fugacity
+    // evaluations (route 6): the general kernel takes them at once, beside
+    // the stages, over llist with its own queue counter
+    int long_evals;             // 0: none
+    const int32_t* prev_evals;  // [nb] ITERS row 5 of the previous flash
+    unsigned* lcount;           // counters, continued: long cells,
+    unsigned* lqueue;           //   their work-queue counter
+    int32_t* llist;             // [nb]
     int32_t* route;             // [nb] 0 general kernel, 1 stable/undecided by stage 1, 2 unstable,
-                                //      3 resumed at SplitNR after the split stage
+                                //      3 resumed at SplitNR after the split stage, 4 resumed at
+                                //      the restart loop after the split-Newton stage, 5 written out,
+                                //      6 long (taken by the general kernel beside the stages)
     int32_t* decided;           // [nb] first trial stage 1 found unstable, 4 none
     int32_t* defer;             // [nb] bit k: trial k queued for its Newton step
     double* Ytd;                // [nb] Yt of the decided trial
@@ -76,16 +91,20 @@ struct StaIO {
     double *nr_Yt, *nr_y;       // [nr_cap], [Nc][nr_cap]
     double* rs_Yt;              // [nb]
     int32_t* rs_cnt;            // [8][nb] counters of the stages: SSMsta, NRsta, evaluations,
-                                //   NR steps, SSMsp, Rachford-Rice, SSM converged
+                                //   d ln f evaluations (ITERS row 6), SSMsp, Rachford-Rice, SSM
+                                //   converged, split Newton steps
     double* rs_v;               // [5][Nc][nb] after the split stage: x0, x1, K, ln phi(x0), ln phi(x1)
     double* rs_nu;              // [nb] nu of the split stage's last Rachford-Rice solve
 };
 // stages of the batched flash for one model on persistent grids of grid[0]
-// (stability SSM loops), grid[1] (stability Newton steps) and grid[2] (split
-// SSM loops) 64-lane blocks
+// (stability SSM loops), grid[1] (stability Newton steps), grid[2] (split
+// SSM loops) and grid[3] (split Newton steps) 64-lane blocks
 hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st);
+// before run_pe_sta: zero the stage counters and, with s.long_evals, mark the
+// long cells and list them
+hipError_t run_pe_sta_prep(const StaIO& s, int64_t nb, hipStream_t st);
 // resident blocks per compute unit of the stability SSM (which = 0), Newton
-// (1) and split (2) stages
+// (1), split (2) and split-Newton (3) stages
 int pe_sta_blocks_per_cu(int NC, int which);
 
 // queue order for the next batched flash (pe.hip): cells by descending count

@@ -345,6 +345,25 @@ OCPX_HD void pe_begin(const DevFluid& f, const PeIO& io, int cell, PeLane<NC>& L
         split_nr_begin(f, L, io, l1);
         return;
     }
+    if (io.sta_route && io.sta_route[cell] == 4) {   // SplitNR did not converge in the split-Newton stage
+        L.cnt[0] = io.rs_cnt[cell];
+        L.cnt[1] = io.rs_cnt[n + cell];
+        L.cnt[5] = io.rs_cnt[2 * n + cell];
+        L.cnt[6] = io.rs_cnt[3 * n + cell];
+        L.cnt[2] = io.rs_cnt[4 * n + cell];
+        L.cnt[4] = io.rs_cnt[5 * n + cell];
+        L.cnt[3] = io.rs_cnt[7 * n + cell];
+        L.cnt[7] = L.cnt[6] - L.cnt[3];
+        L.ssm_conv = io.rs_cnt[6 * n + cell];
+        L.Yt = io.rs_Yt[cell];
+        L.NP = 2;
+        L.stage = 0;
+        L.restarts = 0;
+        L.nr_conv = 0;
+        UNROLL for (int i = 0; i < NC; i++) L.a4[i] = io.rs_v[(size_t)(2 * NC + i) * n + cell];
+        after_split_nr(f, L, io);                  // the restart loop (:558-566)
+        return;
+    }
     UNROLL for (int i = 0; i < NC; i++) L.y[i] = L.z[i];
     L.st = ST_FEED;
 }

@@ -29,6 +29,11 @@
 //                         SSMSP's tolerance or cap.  The general kernel
 //                         resumes every such cell at SplitNR and carries on
 //                         (Newton steps, CheckSplit, the restart loop).
+//   stage 5  pe_sta_snr   (with it) persistent lanes over the cells stage 4
+//                         leaves: SplitNR and CheckSplit, results written
+//                         out; a cell whose Newton steps do not converge
+//                         goes to the general kernel, which resumes it at
+//                         the restart loop.
 //
 // Trials run past an undecided one are speculative: their results are not
 // used when the earlier trial turns out unstable, exactly as the reference
@@ -46,6 +51,9 @@ constexpr int NONE = 4;          // decided: no trial found unstable
 #endif
 #ifndef SPLIT_WAVES
 #define SPLIT_WAVES 2             // the same for stage 4
+#endif
+#ifndef SNR_WAVES
+#define SNR_WAVES 1               // and for stage 5
 #endif
 #define DEV __device__ __forceinline__
 
@@ -177,7 +185,9 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
             const DevFluid& f0 = *opaque(fp);
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
-                if (idx < (unsigned)n) {
+                if (idx < (unsigned)n && s.long_evals > 0 && s.route[io.qorder ? io.qorder[idx] : (int)idx] == 6) {
+                    // a long cell: the general kernel beside the stages has it
+                } else if (idx < (unsigned)n) {
                     cell = io.qorder ? io.qorder[idx] : (int)idx;
                     P = io.P[cell];
                     Tk = io.T[cell] + CONV4;                          // SetVarSet :674
@@ -299,9 +309,6 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 // stage 2: StableNR (:475-521) of a queued trial, one lane each, from its
 // SSM's last Y and Yt.  Jacobian JmatSTA (:523-541) solved as the symmetric
 // matrix of its lower triangle (dsysv_ 'U' on row-major storage, :493).
-// Every Jacobian row and every evaluation reads the fluid constants through
-// its own opaque pointer, so the binary-interaction matrix is not held in
-// registers across the solve.
 template <int NC>
 __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
@@ -337,7 +344,7 @@ __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp,
             double S[NC][NC], rhs[1][NC];
             UNROLL for (int i = 0; i < NC; i++) {
                 double row[NC];
-                lnfugx_row<NC>(*opaque(fp), ec, mx, Zx, y, P, i, row);
+                lnfugx_row<NC>(f, ec, mx, Zx, y, P, i, row);
                 double t = 0;
                 UNROLL for (int j = 0; j < NC; j++) t += y[j] * row[j];
                 UNROLL for (int j = 0; j <= i; j++) {
@@ -354,7 +361,7 @@ __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp,
             Yt = yt;
             const double r = 1 / yt;
             UNROLL for (int i = 0; i < NC; i++) y[i] *= r;
-            eos_mix<NC>(*opaque(fp), ec, y, mx);
+            eos_mix<NC>(f, ec, y, mx);
             eos_lnphi<NC>(ec, mx, ev);
             evals++;
             e2 = 0;
@@ -553,13 +560,185 @@ pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
             s.route[cell] = 3;
             busy = false;
         }
-        append(last, s.gcount, s.glist, cell);
+        if (s.snr) append(last, s.tcount, s.tlist, cell);
+        else append(last, s.gcount, s.glist, cell);
+    }
+}
+
+// stage 5: SplitNR (:805-897, AssembleJmatSP :911-937, CalStepNRsp
+// :939-957) from what stage 4 left, then, when it converges, CheckSplit
+// (:960-1010) and the cell's results; otherwise PhaseSplit's restart loop
+// (:558-566) follows in the general kernel (route 4).  One lane per cell,
+// persistent over the list; the Newton system is solved as the symmetric
+// matrix of its lower triangle (dsysv_ 'U' on row-major storage, :835).
+// The fluid constants come through one opaque pointer per step: with one per
+// Jacobian entry (about 80 per step) this kernel made wrong convergence
+// decisions at Nc = 12 under its register spills, where its steps should
+// match the general kernels'.
+template <int NC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SNR_WAVES)))
+pe_sta_snr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
+{
+    const size_t n = io.nb;
+    const int lane = __lane_id();
+    const unsigned total = *s.tcount;
+    int cell = -1, iter = 0;
+    bool busy = false, drained = false, newton = false, conv = false;
+    double P = 0, nu0 = 0, eNR0 = 0;
+    double z[NC], a1[NC], a2[NC], a3[NC], a5[NC];
+    EosCell<NC> ec;
+    while (true) {
+        while (true) {
+            const bool need = !busy && !drained;
+            const unsigned long long m = __ballot(need);
+            if (!m) break;
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(s.thead, (unsigned)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                if (idx < total) {
+                    const DevFluid& f0 = *opaque(fp);
+                    cell = s.tlist[idx];
+                    P = io.P[cell];
+                    eos_cell<NC>(f0, P, io.T[cell] + CONV4, ec);
+                    double Nt = 0;
+                    UNROLL for (int i = 0; i < NC; i++) { z[i] = io.Ni[(size_t)i * n + cell]; Nt += z[i]; }
+                    UNROLL for (int i = 0; i < NC; i++) z[i] = z[i] / Nt;
+                    nu0 = s.rs_nu[cell];
+                    // SplitNR :807-823: n0 = nu0 x0, resSP = ln(f1 / f0)
+                    double e2 = 0;
+                    UNROLL for (int i = 0; i < NC; i++) {
+                        a2[i] = s.rs_v[(size_t)i * n + cell];
+                        a3[i] = s.rs_v[(size_t)(NC + i) * n + cell];
+                        const double l0 = s.rs_v[(size_t)(3 * NC + i) * n + cell];
+                        const double l1 = s.rs_v[(size_t)(4 * NC + i) * n + cell];
+                        a1[i] = nu0 * a2[i];
+                        a5[i] = log((exp(l1) * a3[i] * P) / (exp(l0) * a2[i] * P));
+                        e2 += a5[i] * a5[i];
+                    }
+                    const double eNR = sqrt(e2);
+                    eNR0 = eNR;
+                    iter = 0;
+                    newton = eNR > f0.NRsp_tol;
+                    conv = !newton && eNR < f0.NRsp_tol;
+                    busy = true;
+                } else {
+                    drained = true;
+                }
+            }
+        }
+        if (!__any(busy)) break;
+        if (!busy) continue;
+
+        bool end = !newton;
+        if (newton) {
+            const DevFluid& f = *opaque(fp);
+            const double nu1 = 1 - nu0;
+            double J[NC][NC], rhs[1][NC];
+            {
+                EosMix<NC> m0, m1;
+                LnfN<NC> d0, d1;
+                eos_mix<NC>(f, ec, a2, m0);
+                eos_mix<NC>(f, ec, a3, m1);
+                lnfn_prep<NC>(ec, m0, nu0, d0);
+                lnfn_prep<NC>(ec, m1, nu1, d1);
+                UNROLL for (int i = 0; i < NC; i++) {
+                    UNROLL for (int k = 0; k <= i; k++) {
+                        const double v = lnfn_entry<NC, false>(f, ec, m0, d0, a2, P, nu0, i, k) +
+                                         lnfn_entry<NC, false>(f, ec, m1, d1, a3, P, nu1, i, k);
+                        J[i][k] = v;
+                        J[k][i] = v;
+                    }
+                    rhs[0][i] = a5[i];
+                }
+            }
+            gauss_solve<NC, 1>(J, rhs);
+            double alpha = 1;                          // CalStepNRsp
+            UNROLL for (int i = 0; i < NC; i++)
+                if (a1[i] + alpha * rhs[0][i] < 0) alpha *= 0.9 * fabs(a1[i] / rhs[0][i]);
+            double nn = 0, en = 0;
+            UNROLL for (int i = 0; i < NC; i++) {
+                const double d = alpha * rhs[0][i];
+                a1[i] += d;
+                nn += a1[i];
+                en += 2 * d * d;
+            }
+            nu0 = nn;
+            const double r0 = 1 / nu0, r1 = 1 / (1 - nu0);
+            UNROLL for (int i = 0; i < NC; i++) {
+                a2[i] = fabs(a1[i] * r0);
+                a3[i] = fabs((z[i] - a1[i]) * r1);
+            }
+            double l0[NC], l1[NC];
+            {
+                EosMix<NC> m;
+                eos_mix<NC>(f, ec, a2, m);
+                eos_lnphi<NC>(ec, m, l0);
+                eos_mix<NC>(f, ec, a3, m);
+                eos_lnphi<NC>(ec, m, l1);
+            }
+            double e2 = 0;
+            UNROLL for (int i = 0; i < NC; i++) {
+                a5[i] = log((exp(l1[i]) * a3[i] * P) / (exp(l0[i]) * a2[i] * P));
+                e2 += a5[i] * a5[i];
+            }
+            const double eNR = sqrt(e2);
+            iter++;
+            if (eNR > eNR0 || iter > f.NRsp_maxIt) end = true;
+            else if (en < 1E-16 * (2.0 * NC) * (2.0 * NC)) { conv = true; end = true; }
+            else if (!(eNR > f.NRsp_tol)) end = true;
+            if (end && eNR < f.NRsp_tol) conv = true;
+        }
+        bool restart = false;
+        if (end) {
+            const int c0 = s.rs_cnt[cell], c1 = s.rs_cnt[n + cell], c5 = s.rs_cnt[2 * n + cell];
+            const int c6 = s.rs_cnt[3 * n + cell], c2 = s.rs_cnt[4 * n + cell], c4 = s.rs_cnt[5 * n + cell];
+            if (conv) {                                // CheckSplit, then the results (PhaseEquilibrium :106-111)
+                int NP = 2;
+                const double nuMax = fmax(nu0, 1 - nu0);
+                double eX = 0;
+                UNROLL for (int i = 0; i < NC; i++) eX += (a2[i] - a3[i]) * (a2[i] - a3[i]);
+                if (!(nuMax < 1 && finite_d(eX)) && (!finite_d(eX) || 1 - nuMax < 1E-3)) NP = 1;
+                io.pe_NP[cell] = NP;
+                io.pe_nu[cell] = NP > 1 ? nu0 : 1.0;
+                UNROLL for (int i = 0; i < NC; i++) {
+                    io.pe_x[(size_t)i * n + cell] = NP > 1 ? a2[i] : z[i];
+                    io.pe_x[(size_t)(NC + i) * n + cell] = NP > 1 ? a3[i] : 0.0;
+                }
+                io.ftype_out[cell] = NP > 1 ? 2 : 0;
+                if (io.iters) {
+                    const int cnt[8] = {c0, c1, c2, iter, c4, c5 + 2 * iter, c6 + 2 * iter, c6 + iter};
+                    UNROLL for (int q = 0; q < 8; q++) io.iters[(size_t)q * n + cell] = cnt[q];
+                }
+                s.route[cell] = 5;
+            } else {                                   // the restart loop, in the general kernel
+                s.rs_cnt[2 * n + cell] = c5 + 2 * iter;
+                s.rs_cnt[3 * n + cell] = c6 + 2 * iter;
+                s.rs_cnt[7 * n + cell] = iter;
+                s.route[cell] = 4;
+                restart = true;
+            }
+            busy = false;
+        }
+        append(restart, s.gcount, s.glist, cell);
     }
 }
 
 __global__ void __launch_bounds__(64) sta_reset(unsigned* c)
 {
-    if (threadIdx.x < 5) c[threadIdx.x] = 0;
+    if (threadIdx.x < 9) c[threadIdx.x] = 0;
+}
+
+// route 6 and the list for the cells whose previous flash took more than
+// long_evals fugacity evaluations, route 0 for the others (stage 1 sets theirs)
+__global__ void __launch_bounds__(256) sta_pick_long(int64_t nb, StaIO s)
+{
+    const int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool isl = cell < nb && s.prev_evals[cell] > s.long_evals;
+    if (cell < nb) s.route[cell] = isl ? 6 : 0;
+    append(isl, s.lcount, s.llist, (int32_t)cell);
 }
 
 template <int NC>
@@ -568,7 +747,8 @@ int blocks_per_cu(int which)
     int b = 0;
     const hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_ssm<NC>, 64, 0)
                          : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_nr<NC>, 64, 0)
-                                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_split<NC>, 64, 0);
+                         : which == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_split<NC>, 64, 0)
+                                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_snr<NC>, 64, 0);
     return e == hipSuccess && b > 0 ? b : 2;
 }
 
@@ -576,18 +756,26 @@ template <int NC>
 hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st)
 {
     if (io.nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(sta_reset, dim3(1), dim3(64), 0, st, s.qhead);
     const long need = (io.nb + 63) / 64;
-    long g[3];
-    for (int q = 0; q < 3; q++) g[q] = grid[q] < need ? grid[q] : need;
+    long g[4];
+    for (int q = 0; q < 4; q++) g[q] = grid[q] < need ? grid[q] : need;
     hipLaunchKernelGGL(pe_sta_ssm<NC>, dim3((unsigned)g[0]), dim3(64), 0, st, f, io, s);
     hipLaunchKernelGGL(pe_sta_nr<NC>, dim3((unsigned)g[1]), dim3(64), 0, st, f, io, s);
     hipLaunchKernelGGL(pe_sta_join<NC>, dim3((unsigned)((io.nb + 255) / 256)), dim3(256), 0, st, f, io, s);
     if (s.split) hipLaunchKernelGGL(pe_sta_split<NC>, dim3((unsigned)g[2]), dim3(64), 0, st, f, io, s);
+    if (s.split && s.snr) hipLaunchKernelGGL(pe_sta_snr<NC>, dim3((unsigned)g[3]), dim3(64), 0, st, f, io, s);
     return hipGetLastError();
 }
 
 }  // namespace sta
+
+hipError_t run_pe_sta_prep(const StaIO& s, int64_t nb, hipStream_t st)
+{
+    hipLaunchKernelGGL(sta::sta_reset, dim3(1), dim3(64), 0, st, s.qhead);
+    if (s.long_evals > 0 && nb > 0)
+        hipLaunchKernelGGL(sta::sta_pick_long, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, nb, s);
+    return hipGetLastError();
+}
 
 hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st)
 {

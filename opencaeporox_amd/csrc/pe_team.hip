@@ -603,6 +603,25 @@ DEV void begin(const DevFluid& f, LaneFluid<NC, T>& lf, const PeIO& io, int cell
         split_nr_begin(f, lf, C, io, l1, ln);
         return;
     }
+    if (io.sta_route && io.sta_route[cell] == 4) {   // SplitNR did not converge in the split-Newton stage
+        C.cnt[0] = io.rs_cnt[cell];
+        C.cnt[1] = io.rs_cnt[n + cell];
+        C.cnt[5] = io.rs_cnt[2 * n + cell];
+        C.cnt[6] = io.rs_cnt[3 * n + cell];
+        C.cnt[2] = io.rs_cnt[4 * n + cell];
+        C.cnt[4] = io.rs_cnt[5 * n + cell];
+        C.cnt[3] = io.rs_cnt[7 * n + cell];
+        C.cnt[7] = C.cnt[6] - C.cnt[3];
+        C.ssm_conv = io.rs_cnt[6 * n + cell];
+        C.Yt = io.rs_Yt[cell];
+        C.NP = 2;
+        C.stage = 0;
+        C.restarts = 0;
+        C.nr_conv = 0;
+        SLOTS C.a4[c] = ln.act[c] ? io.rs_v[(2 * NC + (size_t)(c * T + ln.t)) * n + cell] : 0.0;
+        after_split_nr(f, lf, C, io, ln);           // the restart loop (:558-566)
+        return;
+    }
     SLOTS C.y[c] = C.z[c];
     C.st = ST_FEED;
 }

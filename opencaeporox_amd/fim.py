@@ -165,9 +165,20 @@ class IsoTFIM:
 
     def set_flash_stages(self, on):
         """Stages of the batched flash: 0 (or False) none, 1 the strict stability
-        test in its own kernels, 2 (default, or True) also the split's first SSM pass."""
-        v = 2 if on is True else (0 if on is False else int(on))
+        test in its own kernels, 2 also the split's first SSM pass, 3 (default, or
+        True) also its Newton steps."""
+        v = 3 if on is True else (0 if on is False else int(on))
         self._chk(self.lib.ocpx_set_flash_stages(self.h, v), "flash stages")
+
+    def set_flash_long(self, evals):
+        """Cells over `evals` fugacity evaluations in the previous batched flash
+        run beside the stages from the start (0 = off)."""
+        self._chk(self.lib.ocpx_set_flash_long(self.h, int(evals)), "flash long cells")
+
+    def flash_long(self):
+        v = C.c_int32()
+        self._chk(self.lib.ocpx_get_flash_long(self.h, C.byref(v)), "flash long cells")
+        return v.value
 
     def flash_stages(self):
         v = C.c_int32()

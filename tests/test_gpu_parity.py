@@ -102,11 +102,14 @@ def test_flash_config1(oracle, oracle_fma, mode, lanes):
 
 
 # team sizes built per Nc: 4 always, 8 from Nc = 5, 16 from Nc = 9; the
-# batched mode with the stability and split stages (2), the stability stages
-# alone (1) and none (0)
+# batched mode with every stage (3), the stability and split-SSM stages (2),
+# the stability stages alone (1) and none (0); "long": every stage, on a
+# second flash in which the cells over 150 fugacity evaluations in the first
+# run on the branch beside the stages
 PE_CASES = [(Nc, m, lanes, stages) for Nc in (3, 4, 5, 9, 12)
-            for m, lanes, stages in ((PLAIN, 1, 0), (BATCHED, 1, 2), (BATCHED, 4, 2), (BATCHED, 8, 2),
-                                     (BATCHED, 16, 2), (BATCHED, 4, 1), (BATCHED, 1, 0), (BATCHED, 4, 0))
+            for m, lanes, stages in ((PLAIN, 1, 0), (BATCHED, 1, 3), (BATCHED, 4, 3), (BATCHED, 8, 3),
+                                     (BATCHED, 16, 3), (BATCHED, 4, 2), (BATCHED, 4, 1), (BATCHED, 1, 0),
+                                     (BATCHED, 4, 0), (BATCHED, 1, "long"), (BATCHED, 4, "long"))
             if lanes < 8 or (lanes == 8 and Nc >= 5) or (lanes == 16 and Nc >= 9)]
 
 
@@ -120,7 +123,16 @@ def test_phase_equilibrium_cells(oracle, oracle_fma, Nc, mode, lanes, stages):
     wl, z = W.flash_cells(3000, Nc, seed=44 if Nc == 3 else 10 + Nc, P=np.linspace(5.0, 250.0, 3000))
     dev = IsoTFIM.from_workload(wl)
     dev.set_flash_mapping(lanes)
-    dev.set_flash_stages(stages)
+    if stages == "long":
+        dev.set_flash_stages(3)
+        dev.set_flash_long(0)
+        dev.PhaseEquilibrium(mode)
+        nlong = int(np.sum(dev.get("ITERS").reshape(-1, 8)[:, 5] > 150))
+        print(f"Nc={Nc}: {nlong} cells over 150 evaluations go to the branch")
+        assert nlong > 10
+        dev.set_flash_long(150)
+    else:
+        dev.set_flash_stages(stages)
     dev.PhaseEquilibrium(mode)
     NP, nu, x = dev.pe_result()
     P = wl.state["P"]
@@ -298,8 +310,13 @@ def test_newton_passes_end_to_end(oracle, oracle_fma, mode, lanes):
 
 
 def test_reset_to_last_step_repeats_the_pass():
+    """A pass repeated after ResetToLastTimeStep gives the same bits, with the
+    long-cell branch routed by the counters saved with the accepted step."""
     wl = W.cube_grid()
     dev = IsoTFIM.from_workload(wl)
+    dev.set_flash_long(20)
+    dev.newton_pass(1.0, BATCHED)                # the accepted step's flash and counters
+    assert np.sum(dev.get("ITERS").reshape(-1, 8)[:, 5] > 20) > 10
     dev.UpdateLastTimeStep()
     dev.newton_pass(1.0, BATCHED)
     ref = [dev.get(f) for f in ("PHASEEXIST", "S", "DSEC_DPRI", "ITERS")] + [dev.bsr()[2]]
@@ -365,15 +382,22 @@ def test_queue_order_does_not_change_results(lanes):
 
 
 def test_graph_replay_matches_eager():
+    """Three graph replays equal three eager passes of the same sequence, bit
+    for bit (from the second pass on, the cells that took longest in the
+    previous flash run on the flash's second branch)."""
     wl = W.cube_grid()
     a = IsoTFIM.from_workload(wl)
     b = IsoTFIM.from_workload(wl)
-    a.newton_pass(1.0, BATCHED, graph=False)
-    ref = [a.get(f) for f in ("S", "DSEC_DPRI")] + [a.bsr()[2]]
+    a.set_flash_long(20)
+    b.set_flash_long(20)
     for rep in range(3):
-        b.set_state(wl.state)
-        b.set("SKIP_FLAG", np.zeros(b.nb, np.uint32))
-        b.newton_pass(1.0, BATCHED, graph=True)
-        got = [b.get(f) for f in ("S", "DSEC_DPRI")] + [b.bsr()[2]]
+        for d, graph in ((a, False), (b, True)):
+            d.set_state(wl.state)
+            d.set("SKIP_FLAG", np.zeros(d.nb, np.uint32))
+            d.newton_pass(1.0, BATCHED, graph=graph)
+        if rep:
+            assert np.sum(a.get("ITERS").reshape(-1, 8)[:, 5] > 20) > 10
+        ref = [a.get(f) for f in ("S", "DSEC_DPRI", "ITERS")] + [a.bsr()[2]]
+        got = [b.get(f) for f in ("S", "DSEC_DPRI", "ITERS")] + [b.bsr()[2]]
         for x, y in zip(got, ref):
             assert np.array_equal(x, y), f"replay {rep} differs"
