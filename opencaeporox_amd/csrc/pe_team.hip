@@ -778,6 +778,8 @@ DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cel
     constexpr int L = Cell<NC, T>::L;
     if (++C.steps > MAX_STEPS) { C.cnt[7] |= 0x20000000; C.st = ST_DONE; return; }
     bool eval = true;
+    double ev[L];                      // ln phi of this step's evaluation
+    SLOTS ev[c] = 0;
     switch (C.st) {
     case ST_T_START: {
         const double ck = trial_coef(C.k);
@@ -810,13 +812,27 @@ DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cel
     case ST_T_NR:
         stable_nr_pre(lf, C, ln);
         break;
-    case ST_S_SSM_A:
+    case ST_S_SSM_A: {
+        // one SplitSSM2 iteration per step: the Rachford-Rice split, then
+        // ln phi of both phases side by side (two independent evaluations
+        // the compiler can overlap, which shortens a lone cell's iteration),
+        // phase 0 at x = a2 into a5, phase 1 at a3 into ev and C's mixture
         rr2(f, C, ln);
         update_x_rr(C, ln);
-        SLOTS C.y[c] = C.a2[c];
-        break;
-    case ST_S_SSM_B:
+        double s0[L], A0, B0, Z0;
         SLOTS C.y[c] = C.a3[c];
+        mix<NC, T>(lf, ln, C, C.a2, s0, A0, B0, Z0);
+        mix<NC, T>(lf, ln, C, C.y, C.s, C.A, C.B, C.Z);
+        SLOTS {
+            C.a5[c] = ln.act[c] ? lnphi_of(s0[c], C.Bi[c], A0, B0, Z0) : 0.0;
+            ev[c] = ln.act[c] ? lnphi_of(C.s[c], C.Bi[c], C.A, C.B, C.Z) : 0.0;
+        }
+        C.cnt[5] += 2;
+        C.st = ST_S_SSM_B;
+        eval = false;
+        break;
+    }
+    case ST_S_SSM_B:   // entered only from ST_S_SSM_A in the same step
         break;
     case ST_S_NR:
         split_nr_pre(lf, C, ln);
@@ -826,8 +842,6 @@ DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cel
     }
 
     // common evaluator: ln phi at y
-    double ev[L];
-    SLOTS ev[c] = 0;
     if (eval) {
         mix<NC, T>(lf, ln, C, C.y, C.s, C.A, C.B, C.Z);
         SLOTS ev[c] = ln.act[c] ? lnphi_of(C.s[c], C.Bi[c], C.A, C.B, C.Z) : 0.0;
@@ -896,10 +910,6 @@ DEV void step(const DevFluid& f, const LaneFluid<NC, T>& lf, const PeIO& io, Cel
         C.st = ST_A_ITER;
         break;
     }
-    case ST_S_SSM_A:
-        SLOTS C.a5[c] = ev[c];
-        C.st = ST_S_SSM_B;
-        break;
     case ST_S_SSM_B: {
         C.cnt[2]++;
         C.iter++;
