@@ -268,38 +268,77 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
 {
     constexpr int nc = NCB, B = nc + 1, np = 3, G = 64 / B, REC = np * B * B, RL = aux_len(B, nc);
     constexpr int SV = 24, SX = 24 + 9 * B;
+    // The part of a record every lane of a row reads alike (the phase scalars
+    // and x_ij) is staged in LDS by the row's b lanes with 16-byte loads:
+    // slot 0 the row's own cell (once), slot 1 the other cell of the current
+    // connection.  Staged copy: [0, 24) the scalars, then the record's
+    // [XA, RL) (x_ij from its 16-byte aligned start to the record's end).
+    constexpr int XA = SX & ~1, SR = 24 + (RL - XA), OX = 24 + (SX - XA), NCH = SR / 2;
+    static_assert(RL % 2 == 0 && SR % 2 == 0, "records and staged copies are whole 16-byte chunks");
+    __shared__ __align__(16) double stg[G][2][SR];
     const int lane = threadIdx.x;
     const int g = lane / B, r = lane % B;
+    const int gs = g < G ? g : 0;                      // staging group (the lanes past G stage nothing)
     const size_t nb = c.nb;
     const int64_t blk = xcd_chunk(blockIdx.x, gridDim.x);
     const int64_t p = blk * G + g;                     // position in the row order
     const bool valid = (g < G) && (p < c.nb);
     const int64_t n = valid ? (bsr.order ? (int64_t)bsr.order[p] : p) : 0;
+    auto stage = [&](int slot, int64_t cell) {
+        const double* src = c.aux + (size_t)cell * RL;
+        for (int ch = r; ch < NCH; ch += B) {
+            const int so = ch < 12 ? 2 * ch : XA + 2 * (ch - 12);
+            *reinterpret_cast<double2*>(&stg[gs][slot][2 * ch]) = *reinterpret_cast<const double2*>(src + so);
+        }
+    };
     double absV = -1, absN = -1;
+    double diag[B];
+    double res = 0;
+    int64_t s0 = 0, s1 = 0;
     if (valid) {
         // accumulation (BulkAccumuTerm01::CalResFIM / CaldFdXpFIM), column r
-        double diag[B];
         diag[0] = r == 0 ? c.v[n] * c.poroP[n] - c.vfP[n] : -c.vfi[(size_t)(r - 1) * nb + n];
         UNROLL for (int i = 1; i < B; i++) diag[i] = (r == i) ? 1.0 : 0.0;
-        double res = r == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(r - 1) * nb + n] - c.lNi[(size_t)(r - 1) * nb + n];
-
-        const int64_t s0 = bsr.rowptr[n], s1 = bsr.rowptr[n + 1];
-        for (int64_t s = s0 + 1; s < s1; s++) {
-            const int cc = bsr.nbr_conn[s];
-            const int64_t bI = cn.cb[cc], eI = cn.ce[cc];
-            const bool ownB = (bI == n);
-            const double T = cn.trans[cc];
+        res = r == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(r - 1) * nb + n] - c.lNi[(size_t)(r - 1) * nb + n];
+        s0 = bsr.rowptr[n];
+        s1 = bsr.rowptr[n + 1];
+        stage(0, n);
+    }
+    // every lane runs the wavefront's largest connection count (the block is
+    // one wavefront), so the barriers below are reached by all lanes
+    const int cnt = valid ? (int)(s1 - s0 - 1) : 0;
+    int mc = cnt;
+    for (int o = 32; o > 0; o >>= 1) mc = max(mc, __shfl_xor(mc, o));
+    __syncthreads();
+    for (int k = 0; k < mc; k++) {
+        const bool act = k < cnt;
+        const int64_t s = s0 + 1 + k;
+        int cc = 0;
+        int64_t bI = 0, eI = 0;
+        double T = 0;
+        if (act) {
+            cc = bsr.nbr_conn[s];
+            bI = cn.cb[cc];
+            eI = cn.ce[cc];
+            T = cn.trans[cc];
+        }
+        const bool ownB = (bI == n);
+        if (act) stage(1, ownB ? eI : bI);
+        __syncthreads();
+        if (act) {
+            const double* Lb = stg[gs][ownB ? 0 : 1];
+            const double* Le = stg[gs][ownB ? 1 : 0];
             const double* Ab = c.aux + (size_t)bI * RL;
             const double* Ae = c.aux + (size_t)eI * RL;
-            const double zb = Ab[7], ze = Ae[7];
+            const double zb = Lb[7], ze = Le[7];
             const double dGamma = GRAVF * (zb - ze);
             double rb[nc], re[nc];                     // column r of dF/dX_b and dF/dX_e, rows 1..nc
             UNROLL for (int i = 0; i < nc; i++) { rb[i] = 0; re[i] = 0; }
             double flux = 0;
             UNROLL for (int j = 0; j < np; j++) {
                 // upwinding (OCPConvection01::CalFlux :37-78)
-                const double* Sb = Ab + 8 * j;
-                const double* Se = Ae + 8 * j;
+                const double* Sb = Lb + 8 * j;
+                const double* Se = Le + 8 * j;
                 const bool exb = Sb[5] != 0, exe = Se[5] != 0;
                 if (!exb && !exe) continue;
                 double rho;
@@ -310,12 +349,12 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
                 const bool upIsB = !(dP < 0);
                 if (!(upIsB ? exb : exe)) continue;
                 const bool downExists = upIsB ? exe : exb;
-                const double* AU = upIsB ? Ab : Ae;
+                const double* LU = upIsB ? Lb : Le;
                 const int64_t U = upIsB ? bI : eI;
-                const double xi = AU[8 * j + 2], kr = AU[8 * j + 3], mu = AU[8 * j + 4], lam = AU[8 * j + 6];
+                const double xi = LU[8 * j + 2], kr = LU[8 * j + 3], mu = LU[8 * j + 4], lam = LU[8 * j + 6];
                 const double tJ = T * kr / mu;
                 double x[nc];
-                UNROLL for (int i = 0; i < nc; i++) x[i] = AU[SX + j * nc + i];
+                UNROLL for (int i = 0; i < nc; i++) x[i] = LU[OX + j * nc + i];
                 double xr = 0;                          // component of equation r
                 UNROLL for (int i = 0; i < nc; i++) xr = (r == i + 1) ? x[i] : xr;
                 flux += tJ * dP * xi * xr;
@@ -350,6 +389,9 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
                 }
             }
         }
+        __syncthreads();                          // slot 1 is staged again next connection
+    }
+    if (valid) {
         double* dg = bsr.val + (size_t)s0 * B * B + r;
         UNROLL for (int i = 0; i < B; i++) __builtin_nontemporal_store(diag[i], &dg[i * B]);
         const double out = -res;             // Dscalar(-1) :1170

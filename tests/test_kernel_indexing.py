@@ -125,6 +125,33 @@ def test_assembly_indices_in_bounds(lib, dims):
         rec += [SV + 3 * B * j + q * B + lane for j in range(nph) for q in range(3) for lane in range(B)]
         rec += [SX + j * nc + lane for j in range(nph) for lane in range(nc)]
         assert RL % 2 == 0 and max(rec) < RL and min(rec) >= 0, f"B={B}: record offset {max(rec)} of {RL}"
+        # res_asm's LDS copy of a record: 16-byte chunk ch of the copy comes from record
+        # offset 2ch (the 24 scalars) or XA + 2(ch - 12) (x_ij to the record's end); lane r
+        # of a row group takes chunks r, r + B, ...; the scalars and x_ij are read at
+        # [0, 24) and OX + j nc + i; two records per row group, G groups per block
+        XA = SX & ~1
+        SR, OX = 24 + (RL - XA), 24 + (SX - XA)
+        NCH = SR // 2
+        assert SR % 2 == 0 and XA % 2 == 0
+        src = np.array([2 * ch if ch < 12 else XA + 2 * (ch - 12) for ch in range(NCH)])
+        assert src.min() >= 0 and src.max() + 1 < RL, f"B={B}: staged chunk past the record"
+        dst = np.arange(NCH) * 2
+        assert dst.max() + 1 < SR
+        lanes = sorted(ch for rr in range(B) for ch in range(rr, NCH, B))
+        assert lanes == list(range(NCH)), f"B={B}: chunks not covered once each"
+        staged = np.zeros(RL, bool)                      # record offsets the copy holds, at copy index
+        where = np.full(RL, -1)
+        for ch in range(NCH):
+            for h in range(2):
+                staged[src[ch] + h] = True
+                where[src[ch] + h] = dst[ch] + h
+        read = [8 * j + q for j in range(nph) for q in range(8)] + \
+               [SX + j * nc + i for j in range(nph) for i in range(nc)]
+        assert staged[read].all(), f"B={B}: a read record entry is not staged"
+        assert np.array_equal(where[[SX + j * nc + i for j in range(nph) for i in range(nc)]],
+                              [OX + j * nc + i for j in range(nph) for i in range(nc)])
+        assert np.array_equal(where[:24], np.arange(24))
+        assert G * 2 * SR * 8 + 2048 <= 65536, f"B={B}: LDS per block"
         x_rows = [k * B + lane for k in range(nph) for lane in range(B)]            # dS columns
         x_rows += [(nph + j * nc + i) * B + lane for j in range(nph) for i in range(nc) for lane in range(B)]
         assert max(x_rows) < REC, f"B={B}: dXs/dXp offset {max(x_rows)} of {REC}"
