@@ -302,6 +302,7 @@ int sta_alloc(ocpx_ctx* c)
     s.thead = cnt + 6;
     s.lcount = cnt + 7;
     s.lqueue = cnt + 8;
+    s.nhead = cnt + 9;
     if (!c->st2) {
         if (hipStreamCreate(&c->st2) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)

@@ -55,7 +55,7 @@ int pe_batched_blocks_per_cu(int NC);
 
 // stability stages of the batched flash (pe_sta.hip)
 struct StaIO {
-    unsigned* qhead;            // counters, five consecutive words: stage-1 queue head,
+    unsigned* qhead;            // counters, consecutive words: stage-1 queue head,
     unsigned* gcount;           //   cells for the general kernel,
     unsigned* ncount;           //   queued Newton trials,
     unsigned* scount;           //   cells for the split stage,
@@ -73,7 +73,8 @@ struct StaIO {
     int long_evals;             // 0: none
     const int32_t* prev_evals;  // [nb] ITERS row 5 of the previous flash
     unsigned* lcount;           // counters, continued: long cells,
-    unsigned* lqueue;           //   their work-queue counter
+    unsigned* lqueue;           //   their work-queue counter,
+    unsigned* nhead;            //   the Newton-trial queue head
     int32_t* llist;             // [nb]
     int32_t* route;             // [nb] 0 general kernel, 1 stable/undecided by stage 1, 2 unstable,
                                 //      3 resumed at SplitNR after the split stage, 4 resumed at

@@ -306,75 +306,115 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
     }
 }
 
-// stage 2: StableNR (:475-521) of a queued trial, one lane each, from its
-// SSM's last Y and Yt.  Jacobian JmatSTA (:523-541) solved as the symmetric
-// matrix of its lower triangle (dsysv_ 'U' on row-major storage, :493).
+// stage 2: StableNR (:475-521) of the queued trials, persistent lanes: a lane
+// takes a trial (one wave-aggregated atomic for all idle lanes), evaluates it
+// at its SSM's last Y and Yt, and then does one convergence test and at most
+// one Newton step per trip, so lanes whose trials converge early take new ones
+// while the others iterate.  Jacobian JmatSTA (:523-541) solved as the
+// symmetric matrix of its lower triangle (dsysv_ 'U' on row-major storage,
+// :493).  The queue is complete before the launch (stage 1 filled it), and a
+// lane leaves only when it is drained and the lane holds no trial.
 template <int NC>
 __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
     const unsigned total = *s.ncount;
     const size_t n = io.nb;
-    for (unsigned q = blockIdx.x * 64 + threadIdx.x; q < total; q += gridDim.x * 64) {
-        const DevFluid& f = *opaque(fp);
-        const int cell = s.nr_cell[q];
-        const double P = io.P[cell], Tk = io.T[cell] + CONV4;
-        EosCell<NC> ec;
-        eos_cell<NC>(f, P, Tk, ec);
-        double y[NC], di[NC], res[NC];
-        double Yt = s.nr_Yt[q];
-        UNROLL for (int i = 0; i < NC; i++) {
-            y[i] = s.nr_y[(size_t)i * s.nr_cap + q];
-            di[i] = s.di[(size_t)i * n + cell];
-        }
-        EosMix<NC> mx;
-        double ev[NC];
-        eos_mix<NC>(f, ec, y, mx);
-        eos_lnphi<NC>(ec, mx, ev);
-        int evals = 1;
-        double e2 = 0;
-        UNROLL for (int i = 0; i < NC; i++) {       // resSTA = ln(fug[Id] / (fugSta Yt))
-            res[i] = log((di[i] * P) / ((exp(ev[i]) * y[i] * P) * Yt));
-            e2 += res[i] * res[i];
-        }
-        int iter = 0, flag = 1;
-        while (sqrt(e2) > f.NRsta_tol) {
-            double Zx[NC];
-            const double den = z_den(mx), bc = z_bcoef(mx);
-            UNROLL for (int j = 0; j < NC; j++) Zx[j] = ((mx.B - mx.Z) * (2 * mx.s[j]) + bc * ec.Bi[j]) / den;
-            double S[NC][NC], rhs[1][NC];
-            UNROLL for (int i = 0; i < NC; i++) {
-                double row[NC];
-                lnfugx_row<NC>(f, ec, mx, Zx, y, P, i, row);
-                double t = 0;
-                UNROLL for (int j = 0; j < NC; j++) t += y[j] * row[j];
-                UNROLL for (int j = 0; j <= i; j++) {
-                    const double v = (row[j] - t + 1) / Yt;
-                    S[i][j] = v;
-                    S[j][i] = v;
+    const int lane = __lane_id();
+    bool busy = false, drained = false;
+    unsigned q = 0;
+    int cell = 0, iter = 0, evals = 0, flag = 1;
+    double P = 0, Yt = 0, e2 = 0;
+    EosCell<NC> ec;
+    EosMix<NC> mx;
+    double y[NC], di[NC], res[NC];
+    while (true) {
+        const bool need = !busy && !drained;
+        const unsigned long long m = __ballot(need);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            unsigned base = 0;
+            if (lane == leader) base = atomicAdd(s.nhead, (unsigned)__popcll(m));
+            base = __shfl(base, leader);
+            if (need) {
+                const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                if (idx < total) {
+                    const DevFluid& f = *opaque(fp);
+                    q = idx;
+                    cell = s.nr_cell[q];
+                    P = io.P[cell];
+                    const double Tk = io.T[cell] + CONV4;
+                    eos_cell<NC>(f, P, Tk, ec);
+                    Yt = s.nr_Yt[q];
+                    UNROLL for (int i = 0; i < NC; i++) {
+                        y[i] = s.nr_y[(size_t)i * s.nr_cap + q];
+                        di[i] = s.di[(size_t)i * n + cell];
+                    }
+                    double ev[NC];
+                    eos_mix<NC>(f, ec, y, mx);
+                    eos_lnphi<NC>(ec, mx, ev);
+                    evals = 1;
+                    e2 = 0;
+                    UNROLL for (int i = 0; i < NC; i++) {       // resSTA = ln(fug[Id] / (fugSta Yt))
+                        res[i] = log((di[i] * P) / ((exp(ev[i]) * y[i] * P) * Yt));
+                        e2 += res[i] * res[i];
+                    }
+                    iter = 0;
+                    flag = 1;
+                    busy = true;
+                } else {
+                    drained = true;
                 }
-                rhs[0][i] = res[i];
             }
-            gauss_solve<NC, 1>(S, rhs);
-            const double lYt = Yt;
-            double yt = 0;
-            UNROLL for (int i = 0; i < NC; i++) { y[i] = y[i] * lYt + rhs[0][i]; yt += y[i]; }
-            Yt = yt;
-            const double r = 1 / yt;
-            UNROLL for (int i = 0; i < NC; i++) y[i] *= r;
-            eos_mix<NC>(f, ec, y, mx);
-            eos_lnphi<NC>(ec, mx, ev);
-            evals++;
-            e2 = 0;
-            UNROLL for (int i = 0; i < NC; i++) {
-                res[i] = log((di[i] * P) / ((exp(ev[i]) * y[i] * P) * Yt));
-                e2 += res[i] * res[i];
-            }
-            if (++iter > f.NRsta_maxIt) { flag = 0; break; }
         }
-        s.nr_flag[q] = flag;
-        s.nr_Yt[q] = Yt;
-        s.nr_its[q] = iter;
-        s.nr_evals[q] = evals - 1;              // the re-evaluation at the SSM's Y is not the reference's
+        if (!__any(busy)) break;
+        if (busy) {
+            const DevFluid& f = *opaque(fp);
+            bool fin = false;
+            if (sqrt(e2) > f.NRsta_tol) {
+                double Zx[NC];
+                const double den = z_den(mx), bc = z_bcoef(mx);
+                UNROLL for (int j = 0; j < NC; j++) Zx[j] = ((mx.B - mx.Z) * (2 * mx.s[j]) + bc * ec.Bi[j]) / den;
+                double S[NC][NC], rhs[1][NC];
+                UNROLL for (int i = 0; i < NC; i++) {
+                    double row[NC];
+                    lnfugx_row<NC>(f, ec, mx, Zx, y, P, i, row);
+                    double t = 0;
+                    UNROLL for (int j = 0; j < NC; j++) t += y[j] * row[j];
+                    UNROLL for (int j = 0; j <= i; j++) {
+                        const double v = (row[j] - t + 1) / Yt;
+                        S[i][j] = v;
+                        S[j][i] = v;
+                    }
+                    rhs[0][i] = res[i];
+                }
+                gauss_solve<NC, 1>(S, rhs);
+                const double lYt = Yt;
+                double yt = 0;
+                UNROLL for (int i = 0; i < NC; i++) { y[i] = y[i] * lYt + rhs[0][i]; yt += y[i]; }
+                Yt = yt;
+                const double r = 1 / yt;
+                UNROLL for (int i = 0; i < NC; i++) y[i] *= r;
+                double ev[NC];
+                eos_mix<NC>(f, ec, y, mx);
+                eos_lnphi<NC>(ec, mx, ev);
+                evals++;
+                e2 = 0;
+                UNROLL for (int i = 0; i < NC; i++) {
+                    res[i] = log((di[i] * P) / ((exp(ev[i]) * y[i] * P) * Yt));
+                    e2 += res[i] * res[i];
+                }
+                if (++iter > f.NRsta_maxIt) { flag = 0; fin = true; }
+            } else {
+                fin = true;
+            }
+            if (fin) {
+                s.nr_flag[q] = flag;
+                s.nr_Yt[q] = Yt;
+                s.nr_its[q] = iter;
+                s.nr_evals[q] = evals - 1;          // the re-evaluation at the SSM's Y is not the reference's
+                busy = false;
+            }
+        }
     }
 }
 
@@ -728,7 +768,7 @@ pe_sta_snr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 
 __global__ void __launch_bounds__(64) sta_reset(unsigned* c)
 {
-    if (threadIdx.x < 9) c[threadIdx.x] = 0;
+    if (threadIdx.x < 10) c[threadIdx.x] = 0;
 }
 
 // route 6 and the list for the cells whose previous flash took more than
