@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(64) res_asm(CellIO c, ConnIO cn, BsrIO bsr, do
         res = r == 0 ? c.rockVp[n] - c.vf[n] : c.Ni[(size_t)(r - 1) * nb + n] - c.lNi[(size_t)(r - 1) * nb + n];
         s0 = bsr.rowptr[n];
         s1 = bsr.rowptr[n + 1];
-        stage(0, n);
+        if (s1 - s0 > 1) stage(0, n);
     }
     // every lane runs the wavefront's largest connection count (the block is
     // one wavefront), so the barriers below are reached by all lanes
@@ -586,7 +586,7 @@ hipError_t run_res_asm(int nc, const CellIO& c, const ConnIO& cn, const BsrIO& b
 {
     if (c.nb == 0) return hipSuccess;
     if (!dr) return hipErrorInvalidValue;      // the Jacobian pass always forms the residual too
-    if (dj && !c.aux) return hipErrorInvalidValue;
+    if (dj && !c.aux && cn.nconn > 0) return hipErrorInvalidValue;   // records are read only through connections
 #define X(N) case N: return launch_res_asm<N>(c, cn, bsr, dt, dj, res, bmax, barg, st);
     OCPX_NCB_SWITCH(X)
 #undef X

@@ -232,7 +232,7 @@ CellIO cell_io(ocpx_ctx* c)
     io.dKrdS = D(c, OCPX_F_DKRDS); io.dPcdS = D(c, OCPX_F_DPCDS);
     io.poro = D(c, OCPX_F_PORO); io.poroP = D(c, OCPX_F_POROP); io.rockVp = D(c, OCPX_F_ROCKVP);
     io.v = c->v; io.depth = c->depth; io.poroInit = c->poroInit;
-    io.aux = c->aux;
+    io.aux = c->aux;               // null without connections: no record is ever read
     io.mid = c->mid;
     return io;
 }
@@ -617,7 +617,7 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
             return fail(c, OCPX_E_HIP, "queue order upload");
         }
     }
-    ALLOC(c->aux, (size_t)c->nb * aux_len(c->nc + 1, c->nc));
+    if (c->nconn > 0) ALLOC(c->aux, (size_t)c->nb * aux_len(c->nc + 1, c->nc));
     // BSR in OCPMatrix push order: diagonal first, then per connection in
     // index order, row e gets (e, b) and row b gets (b, e)
     c->nnzb = c->nb + 2 * c->nconn;
