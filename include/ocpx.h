@@ -292,8 +292,8 @@ int  ocpx_autotune_flash(ocpx_ctx* ctx, double* ms);
  * kernel resuming them at SplitNR (:807-823); 3 (default): as 2, and SplitNR
  * with CheckSplit (:805-1010) in a lean kernel too, the mapped kernel taking
  * only the cells whose Newton steps do not converge, at the restart loop
- * (:558-566); 0: the mapped kernel takes every cell.  Contexts of several
- * models always run unstaged. */
+ * (:558-566); 0: the mapped kernel takes every cell.  In a context of several
+ * models every stage reads each cell's own model's fluid. */
 int  ocpx_set_flash_stages(ocpx_ctx* ctx, int on);
 int  ocpx_get_flash_stages(ocpx_ctx* ctx, int32_t* on);
 /* Long cells of the staged batched flash.  No counterpart in the reference;

@@ -337,7 +337,7 @@ int launch_pe(ocpx_ctx* c, int mode, bool branch_ok = true)
         return fail(c, OCPX_E_UNSUPPORTED, "contexts of several models flash in batched mode with a team mapping");
     // single-model batched flash: the strict stability test in stages, then the
     // general kernel on the cells that remain (the split, other flash types)
-    const bool staged = mode == OCPX_FLASH_BATCHED && c->stages && !c->mid && c->nb > 0;
+    const bool staged = mode == OCPX_FLASH_BATCHED && c->stages && c->nb > 0;
     if (staged) {
         int rc = sta_alloc(c);
         if (rc) return rc;
@@ -661,7 +661,7 @@ int ctx_create_impl(int nmodel, const ocpx_fluid* fl, const ocpx_sat* sat, const
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     c->ncu = ncu > 0 ? ncu : 256;
-    for (int q = 0; q < 4; q++) c->sta_grid[q] = c->ncu * pe_sta_blocks_per_cu(c->NC, q);
+    for (int q = 0; q < 4; q++) c->sta_grid[q] = c->ncu * pe_sta_blocks_per_cu(c->NC, q, c->mid != nullptr);
     const int per_cu = pe_batched_blocks_per_cu(c->NC);
     c->grid_waves = (ncu > 0 ? ncu : 256) * (per_cu > 0 ? per_cu : 4);
     for (int q = 0; q < 3; q++) {
@@ -1702,7 +1702,7 @@ int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
     if (c->own_stream == false && c->st == nullptr) return fail(c, OCPX_E_STATE, "graph capture needs a stream");
     if (!c->gexec || c->gdt != dt || c->gmode != mode) {
         destroy_graph(c);
-        if (mode == OCPX_FLASH_BATCHED && c->stages && !c->mid) {   // no allocation inside the capture
+        if (mode == OCPX_FLASH_BATCHED && c->stages) {   // no allocation inside the capture
             int rc = sta_alloc(c);
             if (rc) return rc;
         }
@@ -1775,7 +1775,7 @@ int batch_capture(ocpx_batch* b, double dt)
 {
     batch_drop_graph(b);
     for (ocpx_ctx* c : b->ctx)
-        if (c->stages && !c->mid) {                  // no allocation inside the capture
+        if (c->stages) {                  // no allocation inside the capture
             int rc = sta_alloc(c);
             if (rc) return batch_fail(rc, c->err);
         }

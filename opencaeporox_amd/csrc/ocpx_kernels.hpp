@@ -106,7 +106,7 @@ hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s,
 hipError_t run_pe_sta_prep(const StaIO& s, int64_t nb, hipStream_t st);
 // resident blocks per compute unit of the stability SSM (which = 0), Newton
 // (1), split (2) and split-Newton (3) stages
-int pe_sta_blocks_per_cu(int NC, int which);
+int pe_sta_blocks_per_cu(int NC, int which, bool multi);
 
 // queue order for the next batched flash (pe.hip): cells by descending count
 // of fugacity evaluations in the last one; hist has queue_order_buckets()

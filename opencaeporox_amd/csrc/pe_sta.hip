@@ -37,8 +37,8 @@
 //
 // Trials run past an undecided one are speculative: their results are not
 // used when the earlier trial turns out unstable, exactly as the reference
-// would not have run them.  Only multi-model contexts and the plain mode skip
-// these stages.
+// would not have run them.  The plain mode skips these stages; in a context of
+// several models each cell reads its own model's fluid (MULTI).
 #include "ocpx_dev.hpp"
 #include "ocpx_kernels.hpp"
 
@@ -130,6 +130,16 @@ __device__ __forceinline__ void append(bool want, unsigned* count, int32_t* list
     if (want) list[base + __popcll(m & ((1ull << lane) - 1ull))] = value;
 }
 
+// The fluid of a lane's cell: the one fluid through the opaque pointer, or,
+// in a context of several models (MULTI), the cell's model's entry of the
+// per-model array (a per-lane address, so vector loads).
+template <bool MULTI>
+DEV const DevFluid& fl_of(const DevFluid* fp, const PeIO& io, int cell)
+{
+    if constexpr (MULTI) return fp[io.mid[cell]];
+    else return *opaque(fp);
+}
+
 enum : int { S_IDLE = 0, S_FEED, S_START, S_SSM };
 
 // ln phi at y (src/OCPEoS.cpp:68-124, 199-214 in log form) with the cell's
@@ -160,7 +170,7 @@ DEV void lean_lnphi(const F& f, const double* sA, double PT, const double* y, do
 }
 
 // stage 1: one lane per cell, persistent over the work queue
-template <int NC>
+template <int NC, bool MULTI>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(STA_WAVES)))
 pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
@@ -182,13 +192,13 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
             if (lane == leader) base = atomicAdd(s.qhead, (unsigned)__popcll(m));
             base = __shfl(base, leader);
             bool general = false;
-            const DevFluid& f0 = *opaque(fp);
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
                 if (idx < (unsigned)n && s.long_evals > 0 && s.route[io.qorder ? io.qorder[idx] : (int)idx] == 6) {
                     // a long cell: the general kernel beside the stages has it
                 } else if (idx < (unsigned)n) {
                     cell = io.qorder ? io.qorder[idx] : (int)idx;
+                    const DevFluid& f0 = fl_of<MULTI>(fp, io, cell);
                     P = io.P[cell];
                     Tk = io.T[cell] + CONV4;                          // SetVarSet :674
                     double Ni[NC], Nt = 0;
@@ -220,7 +230,7 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
         if (!__any(st != S_IDLE)) break;
         if (st == S_IDLE) continue;
 
-        CFluid& f = *cfl(fp);
+        auto step = [&](const auto& f) {
         // the composition evaluated in this step (the feed's is in y already)
         if (st == S_START) {       // Y = x / Kw[k] (:420-425), x re-read from Ni
             const double ck = trial_coef(k);
@@ -303,6 +313,9 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
                 st = S_START;
             }
         }
+        };
+        if constexpr (MULTI) step(fp[io.mid[cell]]);
+        else step(*cfl(fp));
     }
 }
 
@@ -314,7 +327,7 @@ pe_sta_ssm(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 // symmetric matrix of its lower triangle (dsysv_ 'U' on row-major storage,
 // :493).  The queue is complete before the launch (stage 1 filled it), and a
 // lane leaves only when it is drained and the lane holds no trial.
-template <int NC>
+template <int NC, bool MULTI>
 __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
     const unsigned total = *s.ncount;
@@ -338,9 +351,9 @@ __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp,
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
                 if (idx < total) {
-                    const DevFluid& f = *opaque(fp);
                     q = idx;
                     cell = s.nr_cell[q];
+                    const DevFluid& f = fl_of<MULTI>(fp, io, cell);
                     P = io.P[cell];
                     const double Tk = io.T[cell] + CONV4;
                     eos_cell<NC>(f, P, Tk, ec);
@@ -368,7 +381,7 @@ __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp,
         }
         if (!__any(busy)) break;
         if (busy) {
-            const DevFluid& f = *opaque(fp);
+            const DevFluid& f = fl_of<MULTI>(fp, io, cell);
             bool fin = false;
             if (sqrt(e2) > f.NRsta_tol) {
                 double Zx[NC];
@@ -420,14 +433,14 @@ __global__ void __launch_bounds__(64) pe_sta_nr(const DevFluid* __restrict__ fp,
 
 // stage 3: the reference's order of trials, the counters up to the deciding
 // trial, and the split queue
-template <int NC>
+template <int NC, bool MULTI>
 __global__ void __launch_bounds__(256) pe_sta_join(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
     const size_t n = io.nb;
     const int64_t cell = (int64_t)blockIdx.x * 256 + threadIdx.x;
     bool split = false;
     if (cell < (int64_t)n && s.route[cell] == 1) {
-        const DevFluid& f = *fp;
+        const DevFluid& f = MULTI ? fp[io.mid[cell]] : *fp;
         const int dec = s.decided[cell], def = s.defer[cell];
         int kstar = NONE;
         double Yt = 0;
@@ -484,7 +497,7 @@ __global__ void __launch_bounds__(256) pe_sta_join(const DevFluid* __restrict__ 
 // sum (f1 / f0 - 1)^2 in the literal form of :629-635, and K = phi1 / phi0.
 // x0, x1 are formed again from z, K and nu wherever they are needed, so a
 // lane carries z, K, sqrt(Ai) and ln phi(x0) across the iteration.
-template <int NC>
+template <int NC, bool MULTI>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SPLIT_WAVES)))
 pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
@@ -507,8 +520,8 @@ pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
                 if (idx < total) {
-                    const DevFluid& f0 = *opaque(fp);
                     cell = s.slist[idx];
+                    const DevFluid& f0 = fl_of<MULTI>(fp, io, cell);
                     P = io.P[cell];
                     const double Tk = io.T[cell] + CONV4, logP = log(P);
                     PT = P / Tk;
@@ -538,7 +551,7 @@ pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
         if (!__any(busy)) break;
         if (!busy) continue;
 
-        const DevFluid& f = *opaque(fp);
+        const DevFluid& f = fl_of<MULTI>(fp, io, cell);
         {   // RachfordRice2 on K
             double Kmin = K[0], Kmax = K[0];
             UNROLL for (int i = 1; i < NC; i++) { Kmin = fmin(Kmin, K[i]); Kmax = fmax(Kmax, K[i]); }
@@ -566,9 +579,11 @@ pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
         // from one evaluation to the next
         double x[NC], l0[NC], l1[NC];
         UNROLL for (int i = 0; i < NC; i++) x[i] = K[i] * (z[i] / (1 + nu * (K[i] - 1)));     // x0
-        lean_lnphi<NC>(*cfl(fp), sA, PT, x, l0);
+        if constexpr (MULTI) lean_lnphi<NC>(fp[io.mid[cell]], sA, PT, x, l0);
+        else lean_lnphi<NC>(*cfl(fp), sA, PT, x, l0);
         UNROLL for (int i = 0; i < NC; i++) x[i] = z[i] / (1 + nu * (K[i] - 1));              // x1
-        lean_lnphi<NC>(*cfl(fp), sA, PT, x, l1);
+        if constexpr (MULTI) lean_lnphi<NC>(fp[io.mid[cell]], sA, PT, x, l1);
+        else lean_lnphi<NC>(*cfl(fp), sA, PT, x, l1);
         iter++;
         double Se = 0;
         UNROLL for (int i = 0; i < NC; i++) {
@@ -615,7 +630,7 @@ pe_sta_split(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 // Jacobian entry (about 80 per step) this kernel made wrong convergence
 // decisions at Nc = 12 under its register spills, where its steps should
 // match the general kernels'.
-template <int NC>
+template <int NC, bool MULTI>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(SNR_WAVES)))
 pe_sta_snr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 {
@@ -639,8 +654,8 @@ pe_sta_snr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
             if (need) {
                 const unsigned idx = base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
                 if (idx < total) {
-                    const DevFluid& f0 = *opaque(fp);
                     cell = s.tlist[idx];
+                    const DevFluid& f0 = fl_of<MULTI>(fp, io, cell);
                     P = io.P[cell];
                     eos_cell<NC>(f0, P, io.T[cell] + CONV4, ec);
                     double Nt = 0;
@@ -674,7 +689,7 @@ pe_sta_snr(const DevFluid* __restrict__ fp, PeIO io, StaIO s)
 
         bool end = !newton;
         if (newton) {
-            const DevFluid& f = *opaque(fp);
+            const DevFluid& f = fl_of<MULTI>(fp, io, cell);
             const double nu1 = 1 - nu0;
             double J[NC][NC], rhs[1][NC];
             {
@@ -781,29 +796,29 @@ __global__ void __launch_bounds__(256) sta_pick_long(int64_t nb, StaIO s)
     append(isl, s.lcount, s.llist, (int32_t)cell);
 }
 
-template <int NC>
+template <int NC, bool MULTI>
 int blocks_per_cu(int which)
 {
     int b = 0;
-    const hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_ssm<NC>, 64, 0)
-                         : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_nr<NC>, 64, 0)
-                         : which == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_split<NC>, 64, 0)
-                                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_snr<NC>, 64, 0);
+    const hipError_t e = which == 0   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_ssm<NC, MULTI>, 64, 0)
+                         : which == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_nr<NC, MULTI>, 64, 0)
+                         : which == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_split<NC, MULTI>, 64, 0)
+                                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, pe_sta_snr<NC, MULTI>, 64, 0);
     return e == hipSuccess && b > 0 ? b : 2;
 }
 
-template <int NC>
+template <int NC, bool MULTI>
 hipError_t launch(const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st)
 {
     if (io.nb == 0) return hipSuccess;
     const long need = (io.nb + 63) / 64;
     long g[4];
     for (int q = 0; q < 4; q++) g[q] = grid[q] < need ? grid[q] : need;
-    hipLaunchKernelGGL(pe_sta_ssm<NC>, dim3((unsigned)g[0]), dim3(64), 0, st, f, io, s);
-    hipLaunchKernelGGL(pe_sta_nr<NC>, dim3((unsigned)g[1]), dim3(64), 0, st, f, io, s);
-    hipLaunchKernelGGL(pe_sta_join<NC>, dim3((unsigned)((io.nb + 255) / 256)), dim3(256), 0, st, f, io, s);
-    if (s.split) hipLaunchKernelGGL(pe_sta_split<NC>, dim3((unsigned)g[2]), dim3(64), 0, st, f, io, s);
-    if (s.split && s.snr) hipLaunchKernelGGL(pe_sta_snr<NC>, dim3((unsigned)g[3]), dim3(64), 0, st, f, io, s);
+    hipLaunchKernelGGL((pe_sta_ssm<NC, MULTI>), dim3((unsigned)g[0]), dim3(64), 0, st, f, io, s);
+    hipLaunchKernelGGL((pe_sta_nr<NC, MULTI>), dim3((unsigned)g[1]), dim3(64), 0, st, f, io, s);
+    hipLaunchKernelGGL((pe_sta_join<NC, MULTI>), dim3((unsigned)((io.nb + 255) / 256)), dim3(256), 0, st, f, io, s);
+    if (s.split) hipLaunchKernelGGL((pe_sta_split<NC, MULTI>), dim3((unsigned)g[2]), dim3(64), 0, st, f, io, s);
+    if (s.split && s.snr) hipLaunchKernelGGL((pe_sta_snr<NC, MULTI>), dim3((unsigned)g[3]), dim3(64), 0, st, f, io, s);
     return hipGetLastError();
 }
 
@@ -820,7 +835,7 @@ hipError_t run_pe_sta_prep(const StaIO& s, int64_t nb, hipStream_t st)
 hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s, const int* grid, hipStream_t st)
 {
     switch (NC) {
-#define OCPX_CASE(N) case N: return sta::launch<N>(f, io, s, grid, st);
+#define OCPX_CASE(N) case N: return io.mid ? sta::launch<N, true>(f, io, s, grid, st) : sta::launch<N, false>(f, io, s, grid, st);
         OCPX_CASE(2) OCPX_CASE(3) OCPX_CASE(4) OCPX_CASE(5) OCPX_CASE(6) OCPX_CASE(7)
         OCPX_CASE(8) OCPX_CASE(9) OCPX_CASE(10) OCPX_CASE(11) OCPX_CASE(12)
 #undef OCPX_CASE
@@ -828,10 +843,10 @@ hipError_t run_pe_sta(int NC, const DevFluid* f, const PeIO& io, const StaIO& s,
     }
 }
 
-int pe_sta_blocks_per_cu(int NC, int which)
+int pe_sta_blocks_per_cu(int NC, int which, bool multi)
 {
     switch (NC) {
-#define OCPX_CASE(N) case N: return sta::blocks_per_cu<N>(which);
+#define OCPX_CASE(N) case N: return multi ? sta::blocks_per_cu<N, true>(which) : sta::blocks_per_cu<N, false>(which);
         OCPX_CASE(2) OCPX_CASE(3) OCPX_CASE(4) OCPX_CASE(5) OCPX_CASE(6) OCPX_CASE(7)
         OCPX_CASE(8) OCPX_CASE(9) OCPX_CASE(10) OCPX_CASE(11) OCPX_CASE(12)
 #undef OCPX_CASE
