@@ -460,9 +460,10 @@ def main():
     else:
         pe_kernel = f"pe_batched<{a.Nc}>"
     # the batched flash is one launch group: the stages, then the mapped kernel
-    pe_group = ([f"pe_sta_ssm<{a.Nc}>", f"pe_sta_nr<{a.Nc}>", f"pe_sta_join<{a.Nc}>"]
-                + ([f"pe_sta_split<{a.Nc}>"] if stages >= 2 else [])
-                + ([f"pe_sta_snr<{a.Nc}>"] if stages >= 3 else []) if stages else []) + [pe_kernel]
+    # the stage kernels of a context of one model (template flag false)
+    pe_group = ([f"pe_sta_ssm<{a.Nc}, false>", f"pe_sta_nr<{a.Nc}, false>", f"pe_sta_join<{a.Nc}, false>"]
+                + ([f"pe_sta_split<{a.Nc}, false>"] if stages >= 2 else [])
+                + ([f"pe_sta_snr<{a.Nc}, false>"] if stages >= 3 else []) if stages else []) + [pe_kernel]
     group_traffic = [measured_traffic(k, traffic_key(a)) for k in pe_group]
     pe_traffic = sum(group_traffic) if all(v is not None for v in group_traffic) else None
     tflops = head["flops"] / (head["kernel_ms"]["pe"] * 1e-3) / 1e12
