@@ -99,6 +99,9 @@ struct ocpx_ctx {
     int long_evals = 1000;       // staged flash: cells over this many evaluations last time run beside the stages
     hipStream_t st2 = nullptr;   // the branch that runs them, forked from and joined to st by events
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t st3 = nullptr;   // the tail of the cells the stages finished, beside the general kernel
+    hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr;
+    bool tail_rest = false;      // the next tail takes only the cells that pass left
     // Peaceman wells (ocpx_set_wells)
     int32_t nwell = 0, nperf = 0;
     ocpx_well* wells = nullptr;  // [nwell] device copy
@@ -308,14 +311,22 @@ int sta_alloc(ocpx_ctx* c)
             hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
             return fail(c, OCPX_E_HIP, "flash branch stream or events");
     }
+    if (!c->st3) {
+        if (hipStreamCreate(&c->st3) != hipSuccess || hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming) != hipSuccess)
+            return fail(c, OCPX_E_HIP, "tail branch stream or events");
+    }
     s.nr_cap = cap;
     c->sta_ready = true;
     return OCPX_OK;
 }
 
-// branch_ok: the long cells may run on the context's second stream (off when
-// the context's own stream is itself a branch of a batch capture)
-int launch_pe(ocpx_ctx* c, int mode, bool branch_ok = true)
+TailIO tail_io(ocpx_ctx* c);
+
+// branch_ok: the long cells may run on the context's second stream and the
+// finished cells' tail on its third (off when the context's own stream is
+// itself a branch of a batch capture); tail_follows: launch_tail comes next
+int launch_pe(ocpx_ctx* c, int mode, bool branch_ok = true, bool tail_follows = false)
 {
     PeIO io{};
     io.nb = c->nb;
@@ -342,7 +353,7 @@ int launch_pe(ocpx_ctx* c, int mode, bool branch_ok = true)
         int rc = sta_alloc(c);
         if (rc) return rc;
     }
-    bool branched = false;
+    bool branched = false, early_tail = false;
     {
         TimerScope t(c, 0);
         if (staged) {
@@ -368,6 +379,19 @@ int launch_pe(ocpx_ctx* c, int mode, bool branch_ok = true)
                 branched = true;
             }
             HIPCHK(c, run_pe_sta(c->NC, c->df, io, c->sta, c->sta_grid, c->st));
+            if (tail_follows && branch_ok) {
+                // the cells the stages finished take their FlashDer tail now, beside
+                // the general kernel, whose few long cells leave the device idle
+                TailIO tio = tail_io(c);
+                tio.route = c->sta.route;
+                tio.route_pass = 1;
+                HIPCHK(c, hipEventRecord(c->ev_fork3, c->st));
+                HIPCHK(c, hipStreamWaitEvent(c->st3, c->ev_fork3, 0));
+                HIPCHK(c, run_tail(c->NC, c->df, tio, c->st3));
+                HIPCHK(c, hipEventRecord(c->ev_join3, c->st3));
+                early_tail = true;
+                c->tail_rest = true;
+            }
             io.qorder = c->sta.glist;
             io.nq = c->sta.gcount;
             io.sta_route = c->sta.route;
@@ -383,6 +407,7 @@ int launch_pe(ocpx_ctx* c, int mode, bool branch_ok = true)
             HIPCHK(c, run_pe(c->NC, c->df, io, mode, c->grid_waves, c->st));
         }
         if (branched) HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join, 0));
+        if (early_tail) HIPCHK(c, hipStreamWaitEvent(c->st, c->ev_join3, 0));
     }
     if (mode == OCPX_FLASH_BATCHED) {
         // the next batched flash takes the cells longest first, by this one's
@@ -402,7 +427,7 @@ int default_lanes(int NC, bool multi)
     return NC <= 8 ? 8 : 16;
 }
 
-int launch_tail(ocpx_ctx* c)
+TailIO tail_io(ocpx_ctx* c)
 {
     TailIO io{};
     io.nb = c->nb;
@@ -418,8 +443,19 @@ int launch_tail(ocpx_ctx* c)
     io.skip_flag = (uint32_t*)c->f[OCPX_F_SKIP_FLAG].d;
     io.skip_minEig = D(c, OCPX_F_SKIP_MINEIG); io.skip_P = D(c, OCPX_F_SKIP_P);
     io.skip_T = D(c, OCPX_F_SKIP_T); io.skip_zi = D(c, OCPX_F_SKIP_ZI);
-    TimerScope t(c, 1);
     io.mid = c->mid;
+    return io;
+}
+
+int launch_tail(ocpx_ctx* c)
+{
+    TailIO io = tail_io(c);
+    if (c->tail_rest) {          // the flash ran the tail of its finished cells already
+        io.route = c->sta.route;
+        io.route_pass = 2;
+        c->tail_rest = false;
+    }
+    TimerScope t(c, 1);
     HIPCHK(c, run_tail(c->NC, c->df, io, c->st));
     return OCPX_OK;
 }
@@ -731,6 +767,9 @@ void ocpx_ctx_destroy(ocpx_ctx* c)
     if (c->st2) hipStreamDestroy(c->st2);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->st3) hipStreamDestroy(c->st3);
+    if (c->ev_fork3) hipEventDestroy(c->ev_fork3);
+    if (c->ev_join3) hipEventDestroy(c->ev_join3);
     delete c;
 }
 
@@ -970,7 +1009,7 @@ int ocpx_get_pe_result(ocpx_ctx* c, int32_t* NP, double* nu, double* x)
 int ocpx_flash_der(ocpx_ctx* c, int mode)
 {
     if (!c || (mode != OCPX_FLASH_PLAIN && mode != OCPX_FLASH_BATCHED)) return fail(c, OCPX_E_ARG, "bad mode");
-    int rc = launch_pe(c, mode);
+    int rc = launch_pe(c, mode, true, true);
     if (rc) return rc;
     rc = launch_tail(c);
     if (rc) return rc;
@@ -1709,7 +1748,7 @@ int ocpx_newton_pass(ocpx_ctx* c, double dt, int mode, int use_graph)
         const bool tm = c->timing;
         c->timing = false;
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-        int rc = launch_pe(c, mode);
+        int rc = launch_pe(c, mode, true, true);
         if (!rc) rc = launch_tail(c);
         if (!rc) rc = launch_krpc_rock(c);
         if (!rc) rc = launch_res_asm(c, dt, true, true);

@@ -132,6 +132,10 @@ struct TailIO {
     uint32_t* skip_flag;
     double *skip_minEig, *skip_P, *skip_T, *skip_zi;
     const int32_t* mid;         // [nb] model of each cell, or null (one model)
+    // tail in two passes around the general flash kernel: with route set, pass 1
+    // takes the cells the flash stages finished (route 1 or 5), pass 2 the others
+    const int32_t* route;
+    int route_pass;
 };
 
 // team mapping (pe_team.hip): T lanes per cell, T = 4, 8 or 16

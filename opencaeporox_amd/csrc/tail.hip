@@ -138,6 +138,10 @@ __global__ void __launch_bounds__(64) flash_tail(const DevFluid* __restrict__ fp
 {
     const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cell >= io.nb) return;
+    if (io.route) {
+        const int r = io.route[cell];
+        if ((r == 1 || r == 5) != (io.route_pass == 1)) return;
+    }
     // one fluid: a uniform pointer; several models: the cell's own fluid
     const DevFluid& f = io.mid ? fp[io.mid[cell]] : *fp;
     const size_t n = io.nb;
