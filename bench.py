@@ -56,6 +56,9 @@ def parse():
                    help="also measure the assembly at b = 7 (Nc = 5) on the same grid")
     p.add_argument("--peaks", type=int, default=1,
                    help="measure achievable copy bandwidth and FP64 DGEMM rate on this GPU")
+    p.add_argument("--queue-order", choices=("auto", "longest-first", "natural"), default="auto",
+                   help="cell order of the flash work queue: the autotuner's pick, or pinned (profiling "
+                        "passes that must see the same dispatches pin it)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="only copy a buffer of this many MiB on the GPU (calibrates the "
                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -440,6 +443,8 @@ def main():
         if mode == BATCHED:
             dev.ResetToLastTimeStep()
             tune = dev.autotune_flash()
+            if a.queue_order != "auto":
+                dev.set_flash_queue_order(a.queue_order == "longest-first")
             log(f"autotuner (ms per mapping, lanes per cell): {tune} -> {dev.flash_mapping()}")
         ms, k = time_steps(dev, mode, dt, a.steps, a.warmup, sync)
         it = dev.get("ITERS")
