@@ -450,12 +450,19 @@ TailIO tail_io(ocpx_ctx* c)
 int launch_tail(ocpx_ctx* c)
 {
     TailIO io = tail_io(c);
-    if (c->tail_rest) {          // the flash ran the tail of its finished cells already
-        io.route = c->sta.route;
-        io.route_pass = 2;
-        c->tail_rest = false;
-    }
     TimerScope t(c, 1);
+    if (c->tail_rest) {
+        // the flash ran the tail of the cells its stages finished (route 1 or 5)
+        // already; every other cell is on the general kernel's list or on the
+        // long cells' list, so the rest of the tail runs over those two lists
+        c->tail_rest = false;
+        for (int k = 0; k < 2; k++) {
+            io.list = k == 0 ? c->sta.glist : c->sta.llist;
+            io.nlist = k == 0 ? c->sta.gcount : c->sta.lcount;
+            HIPCHK(c, run_tail(c->NC, c->df, io, c->st));
+        }
+        return OCPX_OK;
+    }
     HIPCHK(c, run_tail(c->NC, c->df, io, c->st));
     return OCPX_OK;
 }

@@ -136,6 +136,11 @@ struct TailIO {
     // takes the cells the flash stages finished (route 1 or 5), pass 2 the others
     const int32_t* route;
     int route_pass;
+    // with list set, the cells are list[0 .. *nlist) (route is not read): pass 2
+    // runs over the general kernel's and the long cells' lists, packed in
+    // wavefronts instead of scattered over all cells
+    const int32_t* list;
+    const unsigned* nlist;
 };
 
 // team mapping (pe_team.hip): T lanes per cell, T = 4, 8 or 16

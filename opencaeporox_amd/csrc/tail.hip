@@ -136,9 +136,10 @@ OCPX_HD float min_eig(double (&A)[NC][NC])
 template <int NC>
 __global__ void __launch_bounds__(64) flash_tail(const DevFluid* __restrict__ fp, TailIO io)
 {
-    const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= io.nb) return;
-    if (io.route) {
+    const int64_t pos = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (io.list ? pos >= (int64_t)*io.nlist : pos >= io.nb) return;
+    const int64_t cell = io.list ? (int64_t)io.list[pos] : pos;
+    if (!io.list && io.route) {
         const int r = io.route[cell];
         if ((r == 1 || r == 5) != (io.route_pass == 1)) return;
     }
